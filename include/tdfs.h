@@ -1,0 +1,175 @@
+/*
+ * tdfs.h -- C ABI of the MI355X T-DFS subgraph-matching library (libtdfs.so).
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes.  Every function
+ * returns 0 on success or a negative TDFS_E* code; nothing asserts or exits
+ * across this boundary.  The message of the last failure on the calling
+ * thread is available from tdfs_last_error().
+ *
+ * Reference interfaces replaced (lyuheng/tdfs):
+ *   tdfs_pattern_parse / tdfs_pattern_load  STMatch::PatternPreprocessor ctor
+ *                                           src/pattern.h:90-101 (+ readfile 110-143)
+ *   tdfs_plan                               STMatch::Pattern src/pattern.h:43-62
+ *                                           (live fields only) + PatternMultiplicity :69
+ *   tdfs_graph_read_meta / tdfs_graph_read  STMatch::GraphPreprocessor::read_bin_file
+ *                                           src/graph.h:136-193
+ *   tdfs_graph_upload                       GraphPreprocessor::to_gpu src/graph.h:34-50,
+ *                                           build_src_vtx src/graph.h:195-212
+ *   tdfs_count                              the launch + reduction of cu_test.cu:80-105
+ *                                           (kernel _parallel_match, src/gpu_match.cuh:10-13)
+ *   tdfs_count_batch, tdfs_serve_*          no counterpart (batched / served forms)
+ */
+#ifndef TDFS_H_
+#define TDFS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDFS_MAX_K 7 /* PAT_SIZE, src/config.h:11 */
+
+/* status codes */
+#define TDFS_OK 0
+#define TDFS_E_ARG -1       /* bad argument */
+#define TDFS_E_IO -2        /* file missing or short */
+#define TDFS_E_FORMAT -3    /* malformed .g / CSR (unsorted, duplicate, self-loop, asymmetric) */
+#define TDFS_E_HIP -4       /* HIP runtime error */
+#define TDFS_E_OOM -5       /* device or host allocation failed */
+#define TDFS_E_OVERFLOW -6  /* count does not fit in 64 bits */
+#define TDFS_E_BUDGET -7    /* serving admission: request exceeds the memory budget */
+#define TDFS_E_STATE -8     /* call not valid in the current state */
+
+/* condition operators, same values as STMatch::CondOperator (src/pattern.h:21) */
+#define TDFS_LT 0
+#define TDFS_GT 1
+#define TDFS_NE 2
+
+/* Matching plan of one pattern: the live fields of STMatch::Pattern
+ * (src/pattern.h:43-62).  Vertex u is matched at DFS depth u, in file order.
+ * cond[u][j] (j < u) is the symmetry-breaking condition of M(u) against M(j):
+ * LT: M(u) < M(j), GT: M(u) > M(j), NE: M(u) != M(j)  (src/pattern.h:416-448).
+ * shared_lvl[u] = dep >= 2 means C(u) is computed from C(dep) intersected with
+ * N(M(b)) for b in bn_sh[u] (result reuse, src/pattern.h:370-413, 459-480);
+ * -1 means no reuse. */
+typedef struct tdfs_plan {
+  int32_t k;                                /* nnodes, 3..7 */
+  int32_t labeled;                          /* 1: lb semantics, 0: ulb */
+  int32_t sym01;                            /* partial_ori[0][0] == 1: level-0 keeps r < c */
+  int32_t label[TDFS_MAX_K];                /* vertex_labels (all 1 when unlabeled) */
+  int32_t deg[TDFS_MAX_K];                  /* pattern degree */
+  int32_t num_bn[TDFS_MAX_K];               /* backward neighbours (ascending) */
+  int32_t bn[TDFS_MAX_K][TDFS_MAX_K];
+  int32_t cond[TDFS_MAX_K][TDFS_MAX_K];     /* TDFS_LT / TDFS_GT / TDFS_NE for j < u */
+  int32_t shared_lvl[TDFS_MAX_K];
+  int32_t num_bn_sh[TDFS_MAX_K];
+  int32_t bn_sh[TDFS_MAX_K][TDFS_MAX_K];
+  int32_t adj[TDFS_MAX_K][TDFS_MAX_K];      /* pattern adjacency, 0/1 */
+  int32_t npairs;                           /* symmetry-breaking pairs (a,b): M(a) < M(b) */
+  int32_t pairs[21][2];
+  uint64_t multiplicity;                    /* |Aut(P)|, PatternMultiplicity */
+} tdfs_plan;
+
+/* Launch parameters.  Zero fields take defaults (tdfs_tuning_default). */
+typedef struct tdfs_tuning {
+  int32_t waves_per_block;   /* workgroup size in 64-lane waves (default 4) */
+  int32_t blocks_per_cu;     /* resident workgroups per CU for the persistent grid (0 = occupancy max) */
+  int32_t chunk_edges;       /* level-0 directed edges per cursor grab (default 64, max 1024) */
+  int32_t timeout_us;        /* donate remaining work after this long in one task (<0: never) */
+  int32_t split_chunk;       /* iterations per donated task (default 1) */
+  int32_t queue_log2;        /* task ring capacity = 2^queue_log2 (default 20) */
+  int32_t donate_levels;     /* donate only from stack levels < this (default 3) */
+  int32_t reuse;             /* 1: result reuse (default); 2: off (testing) */
+  int32_t lds_list_max;      /* reserved */
+  int32_t reserved[7];
+} tdfs_tuning;
+
+/* Result of one counted pattern. */
+typedef struct tdfs_result {
+  uint64_t raw;          /* sum of per-wave counts, before multiplicity (cu_test.cu:103) */
+  uint64_t count;        /* raw * multiplicity when unlabeled, raw when labeled (cu_test.cu:105) */
+  uint64_t alg_bytes;    /* algorithmic bytes (level 0 + every extension), when stats are on */
+  uint64_t extensions;   /* DFS extensions performed (stats) */
+  uint64_t donated;      /* tasks pushed to the task ring */
+  double kernel_ms;      /* device time of the launch (hipEvents on the library stream) */
+} tdfs_result;
+
+typedef struct tdfs_ctx tdfs_ctx;
+typedef struct tdfs_graph tdfs_graph;
+
+const char *tdfs_last_error(void);
+const char *tdfs_version(void);
+
+/* ---------------- host side: patterns ---------------- */
+/* Parse the text of a .g file (src/pattern.h:110-143) and build its plan. */
+int tdfs_pattern_parse(const char *text, int32_t labeled, tdfs_plan *out);
+/* Same, reading the file at path. */
+int tdfs_pattern_load(const char *path, int32_t labeled, tdfs_plan *out);
+/* Matching-order construction: permutes a connected pattern so that every
+ * vertex u >= 1 has a backward neighbour and vertex 2 is adjacent to vertex 1
+ * (the precondition of the level-0 prune, src/gpu_match.cu:540-548).
+ * order_out[i] = original id placed at position i.  Writes the reordered .g
+ * text into text_out (capacity cap). */
+int tdfs_pattern_reorder(const char *text, int32_t *order_out, char *text_out, size_t cap);
+
+/* ---------------- host side: graphs (reference binary CSR) ---------------- */
+/* meta4 = {n, nnz, vid_size, max_degree} from <prefix>.meta.txt */
+int tdfs_graph_read_meta(const char *prefix, int64_t *meta4);
+/* Reads <prefix>.vertex.bin / .edge.bin (/.label.bin when labels != NULL)
+ * into caller buffers rowptr[n+1], colidx[nnz], labels[n]. */
+int tdfs_graph_read(const char *prefix, int64_t n, int64_t nnz, int64_t *rowptr, int32_t *colidx,
+                    int32_t *labels);
+/* Writes the four files; labels may be NULL (no .label.bin). */
+int tdfs_graph_write(const char *prefix, int64_t n, int64_t nnz, const int64_t *rowptr,
+                     const int32_t *colidx, const int32_t *labels);
+/* Checks strictly ascending lists, no self-loops, symmetry, ids in range. */
+int tdfs_graph_validate(int64_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx);
+
+/* ---------------- host side: seeded generators ---------------- */
+/* Erdos-Renyi G(n, m): m distinct non-loop undirected edges; nnz = 2m. */
+int tdfs_gen_er(int32_t n, int64_t m, uint64_t seed, int64_t *rowptr, int32_t *colidx);
+/* Chung-Lu: weights (i+i0)^(-1/(gamma-1)), i0 set so the largest expected
+ * degree is dmax; endpoints drawn by weight until m distinct non-loop edges.
+ * colidx capacity 2m. */
+int tdfs_gen_chung_lu(int32_t n, int64_t m, double gamma, double dmax, uint64_t seed,
+                      int64_t *rowptr, int32_t *colidx);
+/* labels[v] uniform in [0, nlabels) */
+int tdfs_gen_labels(int32_t n, int32_t nlabels, uint64_t seed, int32_t *labels);
+/* A connected random pattern with k vertices and m edges
+ * (m in [k-1, k(k-1)/2]); labels uniform in [0, nlabels) when nlabels > 0,
+ * all 1 otherwise.  Already in matching order.  Writes .g text. */
+int tdfs_gen_pattern(int32_t k, int32_t m, int32_t nlabels, uint64_t seed, char *text_out,
+                     size_t cap);
+
+/* ---------------- device side ---------------- */
+void tdfs_tuning_default(tdfs_tuning *t);
+int tdfs_ctx_create(int32_t device, tdfs_ctx **out);
+int tdfs_ctx_set_tuning(tdfs_ctx *ctx, const tdfs_tuning *t);
+int tdfs_ctx_get_tuning(const tdfs_ctx *ctx, tdfs_tuning *t);
+/* stats 1: every launch also accumulates alg_bytes / extensions */
+int tdfs_ctx_set_stats(tdfs_ctx *ctx, int32_t on);
+void tdfs_ctx_destroy(tdfs_ctx *ctx);
+/* device properties: out4 = {compute units, wall-clock kHz, max waves resident, reserved} */
+int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4);
+
+/* Copies a validated CSR to HBM (kept resident until tdfs_graph_free).
+ * labels: int32[n] or NULL (unlabeled). */
+int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const int64_t *rowptr,
+                      const int32_t *colidx, const int32_t *labels, tdfs_graph **out);
+void tdfs_graph_free(tdfs_graph *g);
+
+/* One pattern, one launch, over directed edges [edge_begin, edge_end)
+ * (edge_end < 0: all).  Blocks until done. */
+int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
+               int64_t edge_end, tdfs_result *out);
+/* nplans patterns in ONE persistent launch over the resident graph;
+ * out[i] per plan, kernel_ms is the launch's total in every entry. */
+int tdfs_count_batch(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
+                     tdfs_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDFS_H_ */

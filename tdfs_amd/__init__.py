@@ -1,0 +1,201 @@
+"""tdfs_amd -- T-DFS subgraph matching on MI355X.
+
+Python front end over libtdfs.so (include/tdfs.h).  The names mirror the
+reference's host interface (STMatch::GraphPreprocessor, src/graph.h:26-213;
+STMatch::PatternPreprocessor, src/pattern.h:65-720; cu_test.cu's launch and
+reduction): load or generate a CSR graph, plan a pattern, upload the graph
+once, count.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import Plan, Result, Tuning, check, i32p, i64p, lib, TdfsError
+
+__all__ = [
+    "CSRGraph", "Plan", "Result", "Tuning", "TdfsError", "Device", "DeviceGraph",
+    "plan_pattern", "load_pattern", "reorder_pattern", "gen_pattern",
+]
+
+
+@dataclass
+class CSRGraph:
+    """Symmetric CSR: rowptr int64[n+1], colidx int32[nnz] (each list strictly
+    ascending), labels int32[n] or None."""
+
+    rowptr: np.ndarray
+    colidx: np.ndarray
+    labels: np.ndarray | None = None
+
+    @property
+    def n(self) -> int:
+        return int(self.rowptr.shape[0] - 1)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.colidx.shape[0])
+
+    @property
+    def dmax(self) -> int:
+        return int(np.diff(self.rowptr).max()) if self.n else 0
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.rowptr)
+
+    def validate(self) -> None:
+        check(lib().tdfs_graph_validate(self.n, self.nnz, i64p(self.rowptr), i32p(self.colidx)), "validate")
+
+    # --- reference binary format (src/graph.h:136-193) ---
+    @classmethod
+    def read_bin(cls, prefix: str, labeled: bool = False) -> "CSRGraph":
+        meta = np.zeros(4, dtype=np.int64)
+        check(lib().tdfs_graph_read_meta(prefix.encode(), i64p(meta)), "read meta")
+        n, nnz = int(meta[0]), int(meta[1])
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        colidx = np.zeros(max(nnz, 1), dtype=np.int32)
+        labels = np.zeros(max(n, 1), dtype=np.int32) if labeled else None
+        check(lib().tdfs_graph_read(prefix.encode(), n, nnz, i64p(rowptr), i32p(colidx), i32p(labels)), "read graph")
+        return cls(rowptr, colidx[:nnz].copy(), labels[:n].copy() if labeled else None)
+
+    def write_bin(self, prefix: str) -> None:
+        check(lib().tdfs_graph_write(prefix.encode(), self.n, self.nnz, i64p(self.rowptr), i32p(self.colidx),
+                                     i32p(self.labels)), "write graph")
+
+    # --- seeded generators ---
+    @classmethod
+    def erdos_renyi(cls, n: int, m: int, seed: int) -> "CSRGraph":
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        colidx = np.zeros(max(2 * m, 1), dtype=np.int32)
+        check(lib().tdfs_gen_er(n, m, seed, i64p(rowptr), i32p(colidx)), "gen ER")
+        return cls(rowptr, colidx[: 2 * m].copy())
+
+    @classmethod
+    def chung_lu(cls, n: int, m: int, gamma: float, dmax: float, seed: int) -> "CSRGraph":
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        colidx = np.zeros(max(2 * m, 1), dtype=np.int32)
+        check(lib().tdfs_gen_chung_lu(n, m, gamma, dmax, seed, i64p(rowptr), i32p(colidx)), "gen Chung-Lu")
+        return cls(rowptr, colidx[: 2 * m].copy())
+
+    @classmethod
+    def from_edges(cls, n: int, edges) -> "CSRGraph":
+        """Builds a symmetric CSR from undirected edges (duplicates and loops dropped)."""
+        adj = [set() for _ in range(n)]
+        for a, b in edges:
+            if a != b:
+                adj[a].add(b)
+                adj[b].add(a)
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        cols = []
+        for v in range(n):
+            nb = sorted(adj[v])
+            cols.extend(nb)
+            rowptr[v + 1] = rowptr[v] + len(nb)
+        return cls(rowptr, np.asarray(cols, dtype=np.int32))
+
+    def with_labels(self, nlabels: int, seed: int) -> "CSRGraph":
+        labels = np.zeros(max(self.n, 1), dtype=np.int32)
+        check(lib().tdfs_gen_labels(self.n, nlabels, seed, i32p(labels)), "gen labels")
+        return CSRGraph(self.rowptr, self.colidx, labels[: self.n].copy())
+
+
+def plan_pattern(text: str, labeled: bool) -> Plan:
+    """Parses .g text and builds its plan (PatternPreprocessor, src/pattern.h:90-101)."""
+    p = Plan()
+    check(lib().tdfs_pattern_parse(text.encode(), int(labeled), C.byref(p)), "pattern parse")
+    return p
+
+
+def load_pattern(path: str, labeled: bool) -> Plan:
+    p = Plan()
+    check(lib().tdfs_pattern_load(str(path).encode(), int(labeled), C.byref(p)), "pattern load")
+    return p
+
+
+def reorder_pattern(text: str) -> tuple[str, list[int]]:
+    buf = C.create_string_buffer(4096)
+    order = (C.c_int32 * 7)()
+    check(lib().tdfs_pattern_reorder(text.encode(), order, buf, 4096), "pattern reorder")
+    out = buf.value.decode()
+    k = sum(1 for l in out.splitlines() if l.startswith("v"))
+    return out, list(order)[:k]
+
+
+def gen_pattern(k: int, m: int, nlabels: int, seed: int) -> str:
+    buf = C.create_string_buffer(4096)
+    check(lib().tdfs_gen_pattern(k, m, nlabels, seed, buf, 4096), "gen pattern")
+    return buf.value.decode()
+
+
+class DeviceGraph:
+    def __init__(self, dev: "Device", g: CSRGraph):
+        self.dev = dev
+        self.host = g
+        self.handle = C.c_void_p()
+        check(lib().tdfs_graph_upload(dev.handle, g.n, g.nnz, i64p(g.rowptr), i32p(g.colidx), i32p(g.labels),
+                                      C.byref(self.handle)), "graph upload")
+
+    def free(self) -> None:
+        if self.handle:
+            lib().tdfs_graph_free(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Device:
+    """A libtdfs context: one HIP device, one stream, launch workspace."""
+
+    def __init__(self, device: int = 0):
+        self.handle = C.c_void_p()
+        check(lib().tdfs_ctx_create(device, C.byref(self.handle)), "ctx create")
+
+    def info(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        check(lib().tdfs_ctx_info(self.handle, i64p(out)), "ctx info")
+        return {"cus": int(out[0]), "wallclock_khz": int(out[1]), "waves": int(out[2]), "blocks_per_cu": int(out[3])}
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(lib().tdfs_ctx_get_tuning(self.handle, C.byref(t)), "get tuning")
+        return t
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().tdfs_ctx_set_tuning(self.handle, C.byref(t)), "set tuning")
+
+    def set_stats(self, on: bool) -> None:
+        check(lib().tdfs_ctx_set_stats(self.handle, int(on)), "set stats")
+
+    def upload(self, g: CSRGraph) -> DeviceGraph:
+        return DeviceGraph(self, g)
+
+    def count(self, dg: DeviceGraph, plan: Plan, edge_begin: int = 0, edge_end: int = -1) -> Result:
+        r = Result()
+        check(lib().tdfs_count(self.handle, dg.handle, C.byref(plan), edge_begin, edge_end, C.byref(r)), "count")
+        return r
+
+    def count_batch(self, dg: DeviceGraph, plans: list[Plan]) -> list[Result]:
+        arr = (Plan * len(plans))(*plans)
+        res = (Result * len(plans))()
+        check(lib().tdfs_count_batch(self.handle, dg.handle, arr, len(plans), res), "count batch")
+        return list(res)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().tdfs_ctx_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

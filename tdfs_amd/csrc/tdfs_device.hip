@@ -1,0 +1,1113 @@
+// tdfs_device.hip -- the T-DFS counting path on MI355X (gfx950), and the
+// device half of the libtdfs C ABI.
+//
+// Design (one persistent launch per call, any number of patterns):
+//   * One DFS per 64-lane wavefront.  The wave-uniform DFS state (path M,
+//     iterator / end / length per stack level) lives in LDS; candidate lists
+//     live in a per-wave HBM workspace sized to the graph's maximum degree, so
+//     no list is ever truncated (the reference caps lists at 4096,
+//     src/config.h:12).
+//   * Level 0: waves claim chunks of directed edges from one HBM cursor and
+//     filter them 64 at a time (src/gpu_match.cu:496-569 does this on one lane);
+//     survivors are ballot-compacted into the wave's edge list.
+//   * Deeper levels: C(u) = filter(probe) AND membership in every other list,
+//     one pass, 64 candidates per step, ballot + mbcnt compaction
+//     (src/gpu_match.cu:260-494).  The probe is the shortest list; the
+//     symmetry-breaking bounds narrow it by binary search before any lane work.
+//   * Result reuse: C(u) starts from C(shared_lvl[u]) (src/gpu_match.cu:733-776).
+//   * Load balance: a wave that has held one task longer than the timeout while
+//     other waves idle splits the remaining iterations of its shallowest open
+//     level into tasks on an HBM ring (src/gpu_match.cu:863-908, src/queue.h);
+//     a receiver rebuilds the prefix's candidate lists, so reuse stays exact.
+//   * Termination: a wave leaves only when the cursor is spent, the ring is
+//     empty and no wave holds work.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "tdfs.h"
+#include "tdfs_internal.h"
+
+using tdfs::fail;
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kMaxK = TDFS_MAX_K;
+constexpr int kChunkMax = 1024;      // level-0 edges per cursor grab, upper bound
+constexpr int kWavesPerBlock = 4;    // 256-thread workgroups
+constexpr int kBlock = kWavesPerBlock * kWave;
+constexpr long long kSpinMax = 1ll << 24;  // bound on waits for a slot handshake
+
+// Per-pattern plan in the device's layout; read with wave-uniform loads.
+struct DevPlan {
+  int k, labeled, sym01, unsat;
+  int label[8];
+  int deg[8];
+  int num_bn[8];
+  int bn[8][8];
+  int gt_mask[8];  // bit j: M(u) > M(j)
+  int lt_mask[8];  // bit j: M(u) < M(j)
+  int dep[8];      // reuse source vertex, -1 if none
+  int num_bn_sh[8];
+  int bn_sh[8][8];
+};
+
+// A donated unit of work: prefix M(0..p-1) and the iteration range [a, b) of
+// C(p).  fresh = 1 marks level-0 donations whose first extension was never
+// performed by the donor (statistics only).
+struct __attribute__((aligned(64))) TaskRec {
+  int pid, p, a, b;
+  int fresh, pad0, pad1, pad2;
+  int M[8];
+};
+
+// Control words, each on its own 128-byte line.
+struct Ctrl {
+  unsigned long long cursor;
+  long long pad0[15];
+  unsigned long long q_tail;
+  long long pad1[15];
+  unsigned long long q_head;
+  long long pad2[15];
+  long long q_count;  // reserved - released (producer space accounting)
+  long long pad3[15];
+  long long q_avail;  // published - claimed (consumer accounting)
+  long long pad4[15];
+  int busy;
+  int pad5[31];
+  int idle;
+  int pad6[31];
+  int error;
+  int pad7[31];
+};
+
+struct KParams {
+  // graph
+  const long long *rowptr;
+  const int *colidx;
+  const int *degv;
+  const int *src;
+  const int *maxnbr;
+  const unsigned char *labels;
+  // plans and the level-0 cursor space
+  const DevPlan *plans;
+  int nplans;
+  int chunk;
+  long long edge_lo, edge_hi;
+  unsigned long long nchunks;  // chunks per plan
+  unsigned long long total;    // nplans * nchunks * chunk
+  // per-wave workspace
+  int *wbuf;
+  long long wstride;
+  int dstride;
+  // task ring
+  TaskRec *q_data;
+  unsigned long long *q_seq;
+  unsigned long long q_mask;
+  // results: counts[pid]; stats[pid*4 + {0 alg bytes, 1 extensions, 2 donated}]
+  unsigned long long *counts;
+  unsigned long long *stats;
+  Ctrl *ctrl;
+  // tuning
+  long long timeout_ticks;  // < 0: never donate
+  long long deadline_ticks; // a wave idle this long while others stay busy gives up (error)
+  int split_chunk;
+  int donate_levels;
+  int reuse;
+};
+
+struct WaveLds {
+  int M[8];
+  int it[8];
+  int end[8];
+  int len[8];
+};
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ unsigned long long uni64(unsigned long long x) {
+  unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)x);
+  unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned mbcnt64(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_acq(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_rel(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T fadd(T *p, T v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_rlx(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_sc(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// First index i in sorted a[0, n) with a[i] >= x (uniform or per-lane).
+__device__ __forceinline__ int lower_bound_i(const int *a, int n, int x) {
+  int lo = 0;
+  while (n > 0) {
+    const int half = n >> 1;
+    if (a[lo + half] < x) {
+      lo += half + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo;
+}
+
+// Membership of x in sorted a[0, n): branch-free halving, trip count depends
+// only on n.
+__device__ __forceinline__ bool contains(const int *a, int n, int x) {
+  if (n <= 0) return false;
+  const int *base = a;
+  while (n > 1) {
+    const int half = n >> 1;
+    base = (base[half - 1] < x) ? base + half : base;
+    n -= half;
+  }
+  return *base == x;
+}
+
+// ---------------------------------------------------------------------------
+// Candidate generation for pattern vertex u (2 <= u <= k-1) given M(0..u-1).
+// Returns |C(u)|; writes C(u) (ascending) to out unless out == nullptr.
+// ---------------------------------------------------------------------------
+template <bool STATS>
+__device__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int u,
+                           int *__restrict__ out, const int *lbase, bool count_stats,
+                           unsigned long long &balg, unsigned long long &next) {
+  const int lane = lane_id();
+  const int dep = P.reuse ? pl->dep[u] : -1;
+  const int *lp[7];
+  int ln[7];
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    lp[i] = nullptr;
+    ln[i] = 0;
+  }
+  int nl = 0;
+  unsigned long long bytes = 0;
+  if (dep >= 0) {
+    lp[0] = lbase + (long long)(dep - 2) * P.dstride;  // C(dep) is the list of level dep-1
+    ln[0] = S.len[dep - 1];
+    bytes += 4ull * (unsigned)ln[0];
+    const int nsh = pl->num_bn_sh[u];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      if (i < nsh) {
+        const int t = S.M[pl->bn_sh[u][i]];
+        lp[i + 1] = P.colidx + P.rowptr[t];
+        ln[i + 1] = P.degv[t];
+        bytes += 4ull * (unsigned)ln[i + 1] + 16;
+      }
+    }
+    nl = 1 + nsh;
+  } else {
+    const int nb = pl->num_bn[u];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      if (i < nb) {
+        const int t = S.M[pl->bn[u][i]];
+        lp[i] = P.colidx + P.rowptr[t];
+        ln[i] = P.degv[t];
+        bytes += 4ull * (unsigned)ln[i] + 16;
+      }
+    }
+    nl = nb;
+  }
+  // probe = shortest list
+  const int *pp = lp[0];
+  int pn = ln[0];
+  int pidx = 0;
+#pragma unroll
+  for (int i = 1; i < 7; i++) {
+    if (i < nl && ln[i] < pn) {
+      pp = lp[i];
+      pn = ln[i];
+      pidx = i;
+    }
+  }
+  const bool leaf = (u == pl->k - 1);
+  const bool labeled = pl->labeled != 0;
+  if (STATS && count_stats) {
+    if (labeled) bytes += (unsigned)pn;
+  }
+  // filter bounds
+  int lo = -1, hi = INT_MAX;
+  int nev[6];
+  int nemask = 0;
+  const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    nev[j] = -1;
+    if (j < u) {
+      const int m = S.M[j];
+      if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
+      else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
+      else {
+        nev[j] = m;
+        nemask |= 1 << j;
+      }
+    }
+  }
+  int start = 0, stop = pn;
+  if (lo >= 0) start = lower_bound_i(pp, pn, lo + 1);
+  if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
+  unsigned n = 0;
+  if (start < stop) {
+    if (leaf && nl == 1 && !labeled) {
+      // Count-only leaf over one list: |list ∩ (lo,hi)| minus the excluded
+      // path vertices that lie in that window and in the list.
+      n = (unsigned)(stop - start);
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        if ((nemask >> j) & 1) {
+          const int v = nev[j];
+          if (v > lo && v < hi) {
+            const int pos = lower_bound_i(pp + start, stop - start, v);
+            if (pos < stop - start && pp[start + pos] == v) n--;
+          }
+        }
+      }
+    } else {
+      const int lab = pl->label[u];
+      for (int base = start; base < stop; base += kWave) {
+        const int idx = base + lane;
+        bool ok = idx < stop;
+        int x = 0;
+        if (ok) {
+          x = pp[idx];
+#pragma unroll
+          for (int j = 0; j < 6; j++)
+            if ((nemask >> j) & 1) ok = ok && (x != nev[j]);
+          if (labeled && ok) ok = (int)P.labels[x] == lab;
+#pragma unroll
+          for (int i = 0; i < 7; i++)
+            if (i < nl && i != pidx && ok) ok = contains(lp[i], ln[i], x);
+        }
+        const unsigned long long mask = __ballot(ok);
+        if (out) {
+          const unsigned pos = n + mbcnt64(mask);
+          if (ok) out[pos] = x;
+        }
+        n += (unsigned)__popcll(mask);
+      }
+    }
+  }
+  if (STATS && count_stats) {
+    if (!leaf) bytes += 4ull * n;
+    balg += bytes;
+    next += 1;
+  }
+  return n;
+}
+
+// Level-0 filter over directed edges [e0, e1) (src/gpu_match.cu:526-558).
+// "exists v in N(c) with deg(v) >= dP(2)" is maxnbr[c] >= dP(2).
+template <bool STATS>
+__device__ int filter_edges(const KParams &P, const DevPlan *__restrict__ pl, long long e0, long long e1,
+                            int *L0r, int *L0c, unsigned long long &balg) {
+  const int lane = lane_id();
+  const bool labeled = pl->labeled != 0;
+  const int sym01 = pl->sym01;
+  const int l0 = pl->label[0], l1 = pl->label[1];
+  const int d0 = pl->deg[0], d1 = pl->deg[1], d2 = pl->deg[2];
+  unsigned cnt = 0, pre = 0;
+  for (long long base = e0; base < e1; base += kWave) {
+    const long long e = base + lane;
+    bool ok = e < e1;
+    int r = 0, c = 0;
+    if (ok) {
+      r = P.src[e];
+      c = P.colidx[e];
+      if (!labeled) ok = !(sym01 && r >= c);
+      else ok = (int)P.labels[r] == l0 && (int)P.labels[c] == l1;
+    }
+    if (STATS) pre += (unsigned)__popcll(__ballot(ok));
+    if (ok) ok = P.degv[r] >= d0 && P.degv[c] >= d1 && P.maxnbr[c] >= d2;
+    const unsigned long long mask = __ballot(ok);
+    const unsigned pos = cnt + mbcnt64(mask);
+    if (ok) {
+      L0r[pos] = r;
+      L0c[pos] = c;
+    }
+    cnt += (unsigned)__popcll(mask);
+  }
+  if (STATS) balg += 8ull * (unsigned long long)(e1 - e0) + 16ull * pre;
+  return (int)cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Task ring (multi-producer / multi-consumer, bounded, HBM atomics).
+// Producers reserve space (q_count), take consecutive positions (q_tail),
+// write the payload and publish the slot sequence number pos+1, then make the
+// items claimable (q_avail).  Consumers claim an item (q_avail), take a
+// position (q_head), wait for its sequence number, read it, and release the
+// slot for the next lap (sequence pos + capacity).
+// ---------------------------------------------------------------------------
+
+// Reserves n slots; returns the first position or ~0ull when full.  Lane 0
+// performs the accounting; the result is wave-uniform.
+__device__ unsigned long long ring_reserve(const KParams &P, int n) {
+  unsigned long long pos = ~0ull;
+  if (lane_id() == 0) {
+    Ctrl *C = P.ctrl;
+    const long long cap = (long long)P.q_mask + 1;
+    const long long old = fadd(&C->q_count, (long long)n);
+    if (old + n > cap) {
+      fadd(&C->q_count, -(long long)n);
+    } else {
+      pos = fadd(&C->q_tail, (unsigned long long)n);
+    }
+  }
+  return uni64(pos);
+}
+
+// Lane-parallel publication of one slot (called by the lane that owns it).
+__device__ bool ring_publish(const KParams &P, unsigned long long pos, const TaskRec &t) {
+  unsigned long long *seq = P.q_seq + (pos & P.q_mask);
+  long long spins = 0;
+  while (ld_acq(seq) != pos) {
+    if (++spins > kSpinMax) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  P.q_data[pos & P.q_mask] = t;
+  st_rel(seq, pos + 1);
+  return true;
+}
+
+// Claims and reads one task.  Wave-uniform result.
+__device__ bool ring_take(const KParams &P, TaskRec &t) {
+  Ctrl *C = P.ctrl;
+  int got = 0;
+  int err = 0;
+  if (lane_id() == 0) {
+    if (ld_rlx(&C->q_avail) > 0) {
+      const long long old = fadd(&C->q_avail, -1ll);
+      if (old <= 0) {
+        fadd(&C->q_avail, 1ll);
+      } else {
+        const unsigned long long pos = fadd(&C->q_head, 1ull);
+        unsigned long long *seq = P.q_seq + (pos & P.q_mask);
+        long long spins = 0;
+        while (ld_acq(seq) != pos + 1) {
+          if (++spins > kSpinMax) {
+            err = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (!err) {
+          t = P.q_data[pos & P.q_mask];
+          st_rel(seq, pos + P.q_mask + 1);
+          fadd(&C->q_count, -1ll);
+          got = 1;
+        }
+      }
+    }
+    if (err) __hip_atomic_store(&C->error, 2, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  got = uni(got);
+  if (got) {
+    t.pid = uni(t.pid);
+    t.p = uni(t.p);
+    t.a = uni(t.a);
+    t.b = uni(t.b);
+    t.fresh = uni(t.fresh);
+#pragma unroll
+    for (int j = 0; j < 8; j++) t.M[j] = uni(t.M[j]);
+  }
+  return got != 0;
+}
+
+// Donates the untaken iterations of the shallowest open level in
+// [floor, level].  Returns the number of tasks pushed (0 when the ring is
+// full).  All lanes call it.
+__device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
+                      const int *L0c) {
+  int ld = level;
+  for (int l = floor; l <= level; l++) {
+    if (S.it[l] < S.end[l]) {
+      ld = l;
+      break;
+    }
+  }
+  const int it = S.it[ld], end = S.end[ld];
+  const int rem = end - it;
+  if (rem <= 0) return 0;
+  int piece = 1, ntasks = rem;
+  if (ld > 0) {
+    piece = max(P.split_chunk, (rem + 255) / 256);
+    ntasks = (rem + piece - 1) / piece;
+  }
+  const unsigned long long pos0 = ring_reserve(P, ntasks);
+  if (pos0 == ~0ull) return 0;
+  bool okall = true;
+  for (int j = lane_id(); j < ntasks; j += kWave) {
+    TaskRec t;
+    t.pid = pid;
+    t.pad0 = t.pad1 = t.pad2 = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) t.M[q] = 0;
+    if (ld == 0) {
+      t.p = 2;
+      t.M[0] = L0r[it + j];
+      t.M[1] = L0c[it + j];
+      t.a = 0;
+      t.b = INT_MAX;
+      t.fresh = 1;
+    } else {
+      t.p = ld + 1;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (q <= ld) t.M[q] = S.M[q];
+      t.a = it + j * piece;
+      t.b = min(t.a + piece, end);
+      t.fresh = 0;
+    }
+    okall = ring_publish(P, pos0 + j, t) && okall;
+  }
+  if (__ballot(!okall) != 0ull) {
+    if (lane_id() == 0)
+      __hip_atomic_store(&P.ctrl->error, 3, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (lane_id() == 0) fadd(&P.ctrl->q_avail, (long long)ntasks);
+  S.end[ld] = it;  // every lane stores the same value
+  return ntasks;
+}
+
+// Iterative DFS from stack level `floor` (its list, iterator and end already
+// set).  Stack level L iterates C(L+1) (L >= 1) or the edge pairs (L = 0);
+// taking element x at level L fixes M(L+1) = x and computes C(L+2).
+template <bool STATS>
+__device__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
+                    const int *L0r, const int *L0c, int *lbase, unsigned long long &cnt,
+                    unsigned long long &balg, unsigned long long &next, unsigned long long &don) {
+  const int k = pl->k;
+  int level = floor;
+  long long t0 = wall_clock64();
+  const bool may_donate = P.timeout_ticks >= 0;
+  for (;;) {
+    const int it = uni(S.it[level]);
+    const int end = uni(S.end[level]);
+    if (it >= end) {
+      if (level == floor) return;
+      level--;
+      continue;
+    }
+    if (may_donate && level < P.donate_levels) {
+      const long long now = wall_clock64();
+      if (now - t0 > P.timeout_ticks) {
+        t0 = now;
+        int idle = 0;
+        if (lane_id() == 0) idle = ld_rlx(&P.ctrl->idle);
+        idle = uni(idle);
+        if (idle > 0) {
+          const int n = donate(P, S, pid, floor, level, L0r, L0c);
+          don += (unsigned)n;
+          if (n > 0) continue;  // re-read this level: its end may have moved
+        }
+      }
+    }
+    S.it[level] = it + 1;
+    if (level == 0) {
+      S.M[0] = L0r[it];
+      S.M[1] = L0c[it];
+    } else {
+      S.M[level + 1] = lbase[(long long)(level - 1) * P.dstride + it];
+    }
+    const int u = level + 2;
+    if (u == k - 1) {
+      cnt += extend<STATS>(P, pl, S, u, nullptr, lbase, true, balg, next);
+      continue;
+    }
+    int *out = lbase + (long long)level * P.dstride;  // list of level+1
+    const int n = (int)extend<STATS>(P, pl, S, u, out, lbase, true, balg, next);
+    if (n > 0) {
+      S.len[level + 1] = n;
+      S.end[level + 1] = n;
+      S.it[level + 1] = 0;
+      level++;
+    }
+  }
+}
+
+template <bool STATS>
+__global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
+  __shared__ WaveLds lds[kWavesPerBlock];
+  const int lane = lane_id();
+  const int wib = threadIdx.x / kWave;
+  const long long gw = (long long)blockIdx.x * kWavesPerBlock + wib;
+  int *wb = P.wbuf + gw * P.wstride;
+  int *L0r = wb;
+  int *L0c = wb + kChunkMax;
+  int *lbase = wb + 2 * kChunkMax;
+  WaveLds &S = lds[wib];
+  Ctrl *C = P.ctrl;
+  unsigned long long balg = 0, next = 0, don = 0;
+  int stat_pid = -1;
+  bool is_idle = false;
+
+  for (;;) {
+    // ---- acquire work: ring first, then the edge cursor, then wait ----
+    TaskRec t;
+    int kind = 0;
+    long long e0 = 0, e1 = 0;
+    int pid = 0;
+    if (lane == 0) fadd(&C->busy, 1);
+    if (ring_take(P, t)) {
+      kind = 2;
+    } else {
+      unsigned long long pos = 0;
+      if (lane == 0) pos = fadd(&C->cursor, (unsigned long long)P.chunk);
+      pos = uni64(pos);
+      if (pos < P.total) {
+        const unsigned long long c = pos / (unsigned long long)P.chunk;
+        pid = (int)(c % (unsigned long long)P.nplans);
+        const unsigned long long ci = c / (unsigned long long)P.nplans;
+        e0 = P.edge_lo + (long long)ci * P.chunk;
+        e1 = min(e0 + (long long)P.chunk, P.edge_hi);
+        kind = 1;
+      } else {
+        if (lane == 0) fadd(&C->busy, -1);
+        if (!is_idle) {
+          if (lane == 0) fadd(&C->idle, 1);
+          is_idle = true;
+        }
+        const long long t_idle = wall_clock64();
+        int backoff = 1;
+        for (;;) {
+          int stop = 0;
+          if (lane == 0) {
+            if (ld_rlx(&C->error)) stop = 1;
+            else if (ld_sc(&C->q_avail) <= 0 && ld_sc(&C->busy) == 0 && ld_sc(&C->q_avail) <= 0)
+              stop = 1;
+            else if (wall_clock64() - t_idle > P.deadline_ticks) {
+              __hip_atomic_store(&C->error, 4, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+              stop = 1;
+            }
+          }
+          if (uni(stop)) break;
+          // back off: poll every ~2 us at first, up to ~16 us
+          for (int s = 0; s < backoff; s++) __builtin_amdgcn_s_sleep(64);
+          if (backoff < 8) backoff <<= 1;
+          int avail = 0;
+          if (lane == 0) avail = ld_rlx(&C->q_avail) > 0;
+          if (!uni(avail)) continue;
+          if (lane == 0) fadd(&C->busy, 1);
+          if (ring_take(P, t)) {
+            kind = 2;
+            break;
+          }
+          if (lane == 0) fadd(&C->busy, -1);
+        }
+        if (kind == 0) break;
+      }
+    }
+    if (is_idle) {
+      if (lane == 0) fadd(&C->idle, -1);
+      is_idle = false;
+    }
+    if (kind == 2) pid = t.pid;
+    const DevPlan *pl = P.plans + pid;
+    if (STATS && stat_pid != pid) {
+      if (stat_pid >= 0 && lane == 0) {
+        atomicAdd(&P.stats[stat_pid * 4 + 0], balg);
+        atomicAdd(&P.stats[stat_pid * 4 + 1], next);
+      }
+      balg = next = 0;
+      stat_pid = pid;
+    }
+    unsigned long long cnt = 0;
+    if (!pl->unsat) {
+      if (kind == 1) {
+        const int n0 = filter_edges<STATS>(P, pl, e0, e1, L0r, L0c, balg);
+        if (n0 > 0) {
+          if (STATS) next += (unsigned)n0;
+          S.len[0] = n0;
+          S.end[0] = n0;
+          S.it[0] = 0;
+          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, cnt, balg, next, don);
+        }
+      } else {
+        const int p = t.p;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (j < p) S.M[j] = t.M[j];
+        const int k = pl->k;
+        unsigned long long dummy_b = 0, dummy_n = 0;
+        for (int u = 2; u < p; u++) {
+          const int n = (int)extend<STATS>(P, pl, S, u, lbase + (long long)(u - 2) * P.dstride, lbase, false,
+                                           dummy_b, dummy_n);
+          S.len[u - 1] = n;
+        }
+        const bool fresh = t.fresh != 0;
+        if (p == k - 1) {
+          cnt += extend<STATS>(P, pl, S, p, nullptr, lbase, fresh, balg, next);
+        } else {
+          const int n = (int)extend<STATS>(P, pl, S, p, lbase + (long long)(p - 2) * P.dstride, lbase, fresh,
+                                           balg, next);
+          S.len[p - 1] = n;
+          S.it[p - 1] = t.a;
+          S.end[p - 1] = min(t.b, n);
+          dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, cnt, balg, next, don);
+        }
+      }
+    }
+    if (lane == 0) {
+      if (cnt) atomicAdd(&P.counts[pid], cnt);
+      if (don) {
+        atomicAdd(&P.stats[pid * 4 + 2], don);
+        don = 0;
+      }
+      fadd(&C->busy, -1);
+    }
+    don = 0;
+  }
+  if (STATS && stat_pid >= 0 && lane == 0) {
+    atomicAdd(&P.stats[stat_pid * 4 + 0], balg);
+    atomicAdd(&P.stats[stat_pid * 4 + 1], next);
+  }
+}
+
+// Resets the control block, results and ring sequence numbers (a kernel, so
+// that captured launch sequences hold no memset nodes).
+__global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigned long long *stats, int nplans,
+                                 unsigned long long *seq, unsigned long long cap) {
+  const unsigned long long tid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long nth = (unsigned long long)gridDim.x * blockDim.x;
+  int *cw = reinterpret_cast<int *>(ctrl);
+  for (unsigned long long i = tid; i < sizeof(Ctrl) / sizeof(int); i += nth) cw[i] = 0;
+  for (unsigned long long i = tid; i < (unsigned long long)nplans; i += nth) counts[i] = 0;
+  for (unsigned long long i = tid; i < (unsigned long long)nplans * 4; i += nth) stats[i] = 0;
+  for (unsigned long long i = tid; i < cap; i += nth) seq[i] = i;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI (device side)
+// ---------------------------------------------------------------------------
+struct tdfs_graph {
+  int device;
+  int32_t n;
+  int64_t nnz;
+  int32_t dmax;
+  int labeled;
+  long long *rowptr;
+  int *colidx;
+  int *degv;
+  int *src;
+  int *maxnbr;
+  unsigned char *labels;
+};
+
+struct tdfs_ctx {
+  int device;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  int cus;
+  int wallclock_khz;
+  int blocks_per_cu;
+  tdfs_tuning tune;
+  int stats;
+  // buffers grown on demand
+  int *wbuf;
+  size_t wbuf_ints;
+  TaskRec *q_data;
+  unsigned long long *q_seq;
+  unsigned long long q_cap;
+  Ctrl *ctrl;
+  unsigned long long *counts;
+  unsigned long long *stats_buf;
+  DevPlan *plans;
+  int plan_cap;
+};
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(TDFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
+  std::memset(t, 0, sizeof(*t));
+  t->waves_per_block = kWavesPerBlock;
+  t->blocks_per_cu = 0;
+  t->chunk_edges = 64;
+  t->timeout_us = 20;
+  t->split_chunk = 1;
+  t->queue_log2 = 18;
+  t->donate_levels = 3;
+  t->reuse = 1;
+}
+
+static void fill_defaults(tdfs_tuning *t) {
+  tdfs_tuning d;
+  tdfs_tuning_default(&d);
+  if (t->waves_per_block <= 0) t->waves_per_block = d.waves_per_block;
+  if (t->chunk_edges <= 0) t->chunk_edges = d.chunk_edges;
+  if (t->chunk_edges > kChunkMax) t->chunk_edges = kChunkMax;
+  if (t->timeout_us == 0) t->timeout_us = d.timeout_us;
+  if (t->split_chunk <= 0) t->split_chunk = d.split_chunk;
+  if (t->queue_log2 <= 0) t->queue_log2 = d.queue_log2;
+  if (t->queue_log2 < 8) t->queue_log2 = 8;
+  if (t->queue_log2 > 26) t->queue_log2 = 26;
+  if (t->donate_levels <= 0) t->donate_levels = d.donate_levels;
+  if (t->reuse <= 0) t->reuse = d.reuse;
+}
+
+extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
+  if (!out) return fail(TDFS_E_ARG, "null out");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(TDFS_E_ARG, "device %d out of range (%d devices)", device, ndev);
+  HIP_TRY(hipSetDevice(device));
+  tdfs_ctx *c = new tdfs_ctx();
+  std::memset(c, 0, sizeof(*c));
+  c->device = device;
+  tdfs_tuning_default(&c->tune);
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  c->cus = prop.multiProcessorCount;
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+  c->wallclock_khz = khz > 0 ? khz : 100000;
+  int nb = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdfs_match_kernel<false>, kBlock, 0));
+  int nb2 = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, tdfs_match_kernel<true>, kBlock, 0));
+  c->blocks_per_cu = std::max(1, std::min(nb, nb2));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  HIP_TRY(hipMalloc(&c->ctrl, sizeof(Ctrl)));
+  *out = c;
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_ctx_set_tuning(tdfs_ctx *ctx, const tdfs_tuning *t) {
+  if (!ctx || !t) return fail(TDFS_E_ARG, "null argument");
+  ctx->tune = *t;
+  fill_defaults(&ctx->tune);
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_ctx_get_tuning(const tdfs_ctx *ctx, tdfs_tuning *t) {
+  if (!ctx || !t) return fail(TDFS_E_ARG, "null argument");
+  *t = ctx->tune;
+  fill_defaults(t);
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_ctx_set_stats(tdfs_ctx *ctx, int32_t on) {
+  if (!ctx) return fail(TDFS_E_ARG, "null ctx");
+  ctx->stats = on ? 1 : 0;
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4) {
+  if (!ctx || !out4) return fail(TDFS_E_ARG, "null argument");
+  out4[0] = ctx->cus;
+  out4[1] = ctx->wallclock_khz;
+  out4[2] = (int64_t)ctx->cus * ctx->blocks_per_cu * kWavesPerBlock;
+  out4[3] = ctx->blocks_per_cu;
+  return TDFS_OK;
+}
+
+extern "C" void tdfs_ctx_destroy(tdfs_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->wbuf) (void)hipFree(ctx->wbuf);
+  if (ctx->q_data) (void)hipFree(ctx->q_data);
+  if (ctx->q_seq) (void)hipFree(ctx->q_seq);
+  if (ctx->ctrl) (void)hipFree(ctx->ctrl);
+  if (ctx->counts) (void)hipFree(ctx->counts);
+  if (ctx->stats_buf) (void)hipFree(ctx->stats_buf);
+  if (ctx->plans) (void)hipFree(ctx->plans);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const int64_t *rowptr,
+                                 const int32_t *colidx, const int32_t *labels, tdfs_graph **out) {
+  if (!ctx || !rowptr || (!colidx && nnz > 0) || !out || n < 0 || nnz < 0)
+    return fail(TDFS_E_ARG, "bad graph arguments");
+  int rc = tdfs_graph_validate(n, nnz, rowptr, colidx);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::vector<int> degv(n > 0 ? n : 1), src(nnz > 0 ? nnz : 1), maxnbr(n > 0 ? n : 1);
+  int32_t dmax = 0;
+  for (int32_t v = 0; v < n; v++) {
+    degv[v] = (int)(rowptr[v + 1] - rowptr[v]);
+    dmax = std::max(dmax, degv[v]);
+  }
+  for (int32_t v = 0; v < n; v++) {
+    int m = -1;
+    for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) {
+      src[e] = v;
+      m = std::max(m, degv[colidx[e]]);
+    }
+    maxnbr[v] = m;
+  }
+  tdfs_graph *g = new tdfs_graph();
+  std::memset(g, 0, sizeof(*g));
+  g->device = ctx->device;
+  g->n = n;
+  g->nnz = nnz;
+  g->dmax = dmax;
+  g->labeled = labels != nullptr;
+  auto bail = [&](const char *what, hipError_t e) {
+    tdfs_graph_free(g);
+    return fail(TDFS_E_OOM, "%s: %s", what, hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&g->rowptr, sizeof(long long) * (size_t)(n + 1)))) return bail("rowptr", e);
+  if ((e = hipMalloc(&g->colidx, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("colidx", e);
+  if ((e = hipMalloc(&g->degv, sizeof(int) * (size_t)std::max(n, 1)))) return bail("degv", e);
+  if ((e = hipMalloc(&g->src, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("src", e);
+  if ((e = hipMalloc(&g->maxnbr, sizeof(int) * (size_t)std::max(n, 1)))) return bail("maxnbr", e);
+  if ((e = hipMalloc(&g->labels, (size_t)std::max(n, 1)))) return bail("labels", e);
+  std::vector<unsigned char> lab8(n > 0 ? n : 1, 255);
+  if (labels)
+    for (int32_t v = 0; v < n; v++) lab8[v] = (labels[v] >= 0 && labels[v] < 255) ? (unsigned char)labels[v] : 255;
+  if ((e = hipMemcpy(g->rowptr, rowptr, sizeof(long long) * (size_t)(n + 1), hipMemcpyHostToDevice)))
+    return bail("copy rowptr", e);
+  if (nnz > 0) {
+    if ((e = hipMemcpy(g->colidx, colidx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice)))
+      return bail("copy colidx", e);
+    if ((e = hipMemcpy(g->src, src.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice)))
+      return bail("copy src", e);
+  }
+  if (n > 0) {
+    if ((e = hipMemcpy(g->degv, degv.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
+      return bail("copy degv", e);
+    if ((e = hipMemcpy(g->maxnbr, maxnbr.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
+      return bail("copy maxnbr", e);
+    if ((e = hipMemcpy(g->labels, lab8.data(), (size_t)n, hipMemcpyHostToDevice))) return bail("copy labels", e);
+  }
+  *out = g;
+  return TDFS_OK;
+}
+
+extern "C" void tdfs_graph_free(tdfs_graph *g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->rowptr) (void)hipFree(g->rowptr);
+  if (g->colidx) (void)hipFree(g->colidx);
+  if (g->degv) (void)hipFree(g->degv);
+  if (g->src) (void)hipFree(g->src);
+  if (g->maxnbr) (void)hipFree(g->maxnbr);
+  if (g->labels) (void)hipFree(g->labels);
+  delete g;
+}
+
+static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
+  std::memset(&d, 0, sizeof(d));
+  d.k = p.k;
+  d.labeled = p.labeled;
+  d.sym01 = p.sym01;
+  d.unsat = 0;
+  for (int u = 0; u < p.k; u++) {
+    d.label[u] = p.label[u];
+    if (p.labeled && (p.label[u] < 0 || p.label[u] >= 255)) d.unsat = 1;  // no graph vertex carries it
+    d.deg[u] = p.deg[u];
+    d.num_bn[u] = p.num_bn[u];
+    for (int i = 0; i < p.num_bn[u]; i++) d.bn[u][i] = p.bn[u][i];
+    d.gt_mask[u] = d.lt_mask[u] = 0;
+    for (int j = 0; j < u; j++) {
+      if (p.cond[u][j] == TDFS_GT) d.gt_mask[u] |= 1 << j;
+      if (p.cond[u][j] == TDFS_LT) d.lt_mask[u] |= 1 << j;
+    }
+    d.dep[u] = p.shared_lvl[u] >= 2 ? p.shared_lvl[u] : -1;
+    d.num_bn_sh[u] = d.dep[u] >= 0 ? p.num_bn_sh[u] : 0;
+    for (int i = 0; i < d.num_bn_sh[u]; i++) d.bn_sh[u][i] = p.bn_sh[u][i];
+  }
+}
+
+static int check_plan(const tdfs_plan &p) {
+  if (p.k < 3 || p.k > kMaxK) return fail(TDFS_E_ARG, "plan k=%d outside [3, %d]", p.k, kMaxK);
+  for (int u = 1; u < p.k; u++) {
+    if (p.num_bn[u] < 1 || p.num_bn[u] > u) return fail(TDFS_E_ARG, "vertex %d has no backward neighbour", u);
+    for (int i = 0; i < p.num_bn[u]; i++)
+      if (p.bn[u][i] < 0 || p.bn[u][i] >= u) return fail(TDFS_E_ARG, "bad backward neighbour");
+    const int dep = p.shared_lvl[u];
+    if (dep >= 0) {
+      if (dep < 2 || dep >= u) return fail(TDFS_E_ARG, "bad shared_lvl[%d]=%d", u, dep);
+      if (p.num_bn_sh[u] < 0 || p.num_bn_sh[u] > 6) return fail(TDFS_E_ARG, "bad num_bn_sh");
+      for (int i = 0; i < p.num_bn_sh[u]; i++)
+        if (p.bn_sh[u][i] < 0 || p.bn_sh[u][i] >= u) return fail(TDFS_E_ARG, "bad bn_sh");
+    }
+  }
+  return TDFS_OK;
+}
+
+static int ensure_buffers(tdfs_ctx *ctx, const tdfs_graph *g, int nplans, long long waves, long long wstride) {
+  const size_t need = (size_t)waves * (size_t)wstride;
+  if (need > ctx->wbuf_ints) {
+    if (ctx->wbuf) HIP_TRY(hipFree(ctx->wbuf));
+    ctx->wbuf = nullptr;
+    ctx->wbuf_ints = 0;
+    hipError_t e = hipMalloc(&ctx->wbuf, need * sizeof(int));
+    if (e != hipSuccess) return fail(TDFS_E_OOM, "workspace of %zu bytes: %s", need * sizeof(int), hipGetErrorString(e));
+    ctx->wbuf_ints = need;
+  }
+  const unsigned long long cap = 1ull << ctx->tune.queue_log2;
+  if (cap != ctx->q_cap) {
+    if (ctx->q_data) HIP_TRY(hipFree(ctx->q_data));
+    if (ctx->q_seq) HIP_TRY(hipFree(ctx->q_seq));
+    ctx->q_data = nullptr;
+    ctx->q_seq = nullptr;
+    HIP_TRY(hipMalloc(&ctx->q_data, sizeof(TaskRec) * cap));
+    HIP_TRY(hipMalloc(&ctx->q_seq, sizeof(unsigned long long) * cap));
+    ctx->q_cap = cap;
+  }
+  if (nplans > ctx->plan_cap) {
+    if (ctx->plans) HIP_TRY(hipFree(ctx->plans));
+    if (ctx->counts) HIP_TRY(hipFree(ctx->counts));
+    if (ctx->stats_buf) HIP_TRY(hipFree(ctx->stats_buf));
+    ctx->plans = nullptr;
+    ctx->counts = nullptr;
+    ctx->stats_buf = nullptr;
+    HIP_TRY(hipMalloc(&ctx->plans, sizeof(DevPlan) * nplans));
+    HIP_TRY(hipMalloc(&ctx->counts, sizeof(unsigned long long) * nplans));
+    HIP_TRY(hipMalloc(&ctx->stats_buf, sizeof(unsigned long long) * 4 * nplans));
+    ctx->plan_cap = nplans;
+  }
+  (void)g;
+  return TDFS_OK;
+}
+
+static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int nplans, int64_t edge_begin,
+                     int64_t edge_end, tdfs_result *out) {
+  if (!ctx || !g || !plans || !out || nplans < 1) return fail(TDFS_E_ARG, "bad arguments");
+  if (g->device != ctx->device) return fail(TDFS_E_ARG, "graph and context on different devices");
+  for (int i = 0; i < nplans; i++) {
+    int rc = check_plan(plans[i]);
+    if (rc) return rc;
+    if (plans[i].labeled && !g->labeled) return fail(TDFS_E_ARG, "labeled plan on a graph uploaded without labels");
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  fill_defaults(&ctx->tune);
+  if (edge_begin < 0) edge_begin = 0;
+  if (edge_end < 0 || edge_end > g->nnz) edge_end = g->nnz;
+  if (edge_begin > edge_end) edge_begin = edge_end;
+  const int chunk = ctx->tune.chunk_edges;
+  const long long span = edge_end - edge_begin;
+  const unsigned long long nchunks = (unsigned long long)((span + chunk - 1) / chunk);
+  const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
+  const long long wstride = 2ll * kChunkMax + (long long)(kMaxK - 3) * dstride;
+  int bpc = ctx->tune.blocks_per_cu > 0 ? std::min(ctx->tune.blocks_per_cu, ctx->blocks_per_cu) : ctx->blocks_per_cu;
+  const int grid = ctx->cus * bpc;
+  const long long waves = (long long)grid * kWavesPerBlock;
+  int rc = ensure_buffers(ctx, g, nplans, waves, wstride);
+  if (rc) return rc;
+
+  std::vector<DevPlan> dp(nplans);
+  for (int i = 0; i < nplans; i++) to_dev_plan(plans[i], dp[i]);
+  HIP_TRY(hipMemcpyAsync(ctx->plans, dp.data(), sizeof(DevPlan) * nplans, hipMemcpyHostToDevice, ctx->stream));
+
+  KParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.rowptr = g->rowptr;
+  P.colidx = g->colidx;
+  P.degv = g->degv;
+  P.src = g->src;
+  P.maxnbr = g->maxnbr;
+  P.labels = g->labels;
+  P.plans = ctx->plans;
+  P.nplans = nplans;
+  P.chunk = chunk;
+  P.edge_lo = edge_begin;
+  P.edge_hi = edge_end;
+  P.nchunks = nchunks;
+  P.total = (unsigned long long)nplans * nchunks * (unsigned long long)chunk;
+  P.wbuf = ctx->wbuf;
+  P.wstride = wstride;
+  P.dstride = dstride;
+  P.q_data = ctx->q_data;
+  P.q_seq = ctx->q_seq;
+  P.q_mask = ctx->q_cap - 1;
+  P.counts = ctx->counts;
+  P.stats = ctx->stats_buf;
+  P.ctrl = ctx->ctrl;
+  const long long ticks_per_us = std::max(1, ctx->wallclock_khz / 1000);
+  P.timeout_ticks = ctx->tune.timeout_us < 0 ? -1 : (long long)ctx->tune.timeout_us * ticks_per_us;
+  P.deadline_ticks = 300ll * 1000 * 1000 * ticks_per_us;  // 5 minutes
+  P.split_chunk = ctx->tune.split_chunk;
+  P.donate_levels = ctx->tune.donate_levels;
+  P.reuse = ctx->tune.reuse == 1 ? 1 : 0;
+
+  const int init_blocks = (int)std::min<unsigned long long>(1024, (ctx->q_cap + 255) / 256);
+  hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->ctrl, ctx->counts,
+                     ctx->stats_buf, nplans, ctx->q_seq, ctx->q_cap);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  if (ctx->stats)
+    hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, P);
+  else
+    hipLaunchKernelGGL(tdfs_match_kernel<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, P);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  std::vector<unsigned long long> counts(nplans), stats(4 * (size_t)nplans);
+  Ctrl ctrl;
+  HIP_TRY(hipMemcpyAsync(counts.data(), ctx->counts, sizeof(unsigned long long) * nplans, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipMemcpyAsync(stats.data(), ctx->stats_buf, sizeof(unsigned long long) * 4 * nplans,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(&ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ctrl.error) return fail(TDFS_E_HIP, "device scheduler error %d (ring handshake or deadline)", ctrl.error);
+  for (int i = 0; i < nplans; i++) {
+    tdfs_result &r = out[i];
+    std::memset(&r, 0, sizeof(r));
+    r.raw = counts[i];
+    r.alg_bytes = stats[4 * i + 0];
+    r.extensions = stats[4 * i + 1];
+    r.donated = stats[4 * i + 2];
+    r.kernel_ms = ms;
+    if (plans[i].labeled) {
+      r.count = r.raw;
+    } else {
+      const unsigned __int128 prod = (unsigned __int128)r.raw * plans[i].multiplicity;
+      if (prod >> 64) return fail(TDFS_E_OVERFLOW, "count of plan %d overflows 64 bits", i);
+      r.count = (uint64_t)prod;
+    }
+  }
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
+                          int64_t edge_end, tdfs_result *out) {
+  return run_plans(ctx, g, plan, 1, edge_begin, edge_end, out);
+}
+
+extern "C" int tdfs_count_batch(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
+                                tdfs_result *out) {
+  return run_plans(ctx, g, plans, nplans, -1, -1, out);
+}

@@ -1,0 +1,203 @@
+"""GPU parity: libtdfs on cuda:0 against the oracle, bit-exact.
+
+Every case prints its name before touching the GPU so that a failing run
+names it."""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import tdfs_amd as T
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = T.Device(0)
+    yield d
+    d.close()
+
+
+def fixture(i):
+    return (GOLDEN / "patterns" / f"{i}.g").read_text()
+
+
+def check(dev, dg, G, text, labeled, stats=True, **tune):
+    p = T.plan_pattern(text, labeled)
+    q = O.plan(text, labeled)
+    L = G.labels if labeled else None
+    want = O.count(G.rowptr, G.colidx, L, q, threads=4)
+    dev.set_stats(stats)
+    if tune:
+        dev.set_tuning(**tune)
+    r = dev.count(dg, p)
+    assert r.raw == want["raw"], (text, labeled, tune)
+    assert r.count == want["count"]
+    if stats:
+        assert r.alg_bytes == want["alg_bytes"], (text, labeled, tune)
+        assert r.extensions == want["extensions"], (text, labeled, tune)
+    return r
+
+
+def test_smoke_triangle(dev):
+    say("triangle ER(2000, 10000)")
+    G = T.CSRGraph.erdos_renyi(2000, 10000, 1)
+    dg = dev.upload(G)
+    check(dev, dg, G, "v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n", 0)
+
+
+def test_fixtures_small_graphs(dev):
+    graphs = {
+        "er": T.CSRGraph.erdos_renyi(3000, 24000, 5).with_labels(4, 1),
+        "cl": T.CSRGraph.chung_lu(20000, 80000, 2.1, 1200, 6).with_labels(4, 2),
+    }
+    for gname, G in graphs.items():
+        dg = dev.upload(G)
+        for i in range(1, 15):
+            for labeled in (0, 1):
+                say(gname, f"{i}.g", "labeled" if labeled else "unlabeled")
+                check(dev, dg, G, fixture(i), labeled)
+        dg.free()
+
+
+def test_networkx_fixtures(dev):
+    d = json.loads((GOLDEN / "nx_counts.json").read_text())
+    for g in d["graphs"]:
+        G = T.CSRGraph.from_edges(g["n"], g["edges"])
+        G.labels = np.asarray(g["labels"], dtype=np.int32)
+        dg = dev.upload(G)
+        for key, want in g["counts"].items():
+            pname, labeled = key.split(":")
+            say("nx", g["name"], key)
+            p = T.plan_pattern(d["patterns"][pname], int(labeled))
+            assert dev.count(dg, p).count == want, (g["name"], key)
+        dg.free()
+
+
+def test_random_patterns(dev):
+    G = T.CSRGraph.chung_lu(8000, 40000, 2.2, 500, 11).with_labels(3, 4)
+    dg = dev.upload(G)
+    rng = np.random.default_rng(12)
+    for j in range(40):
+        k = int(rng.integers(3, 8))
+        m = int(rng.integers(max(k - 1, k * (k - 1) // 2 - 4), k * (k - 1) // 2 + 1))
+        text = T.gen_pattern(k, m, 3, int(rng.integers(1 << 62)))
+        for labeled in (0, 1):
+            say("random", j, k, m, labeled)
+            check(dev, dg, G, text, labeled)
+
+
+def test_scheduling_modes_are_exact(dev):
+    """Forced scheduler paths: donate at every check, a tiny task ring (full
+    ring path), odd chunk sizes, reuse off; counts and algorithmic bytes are
+    unchanged."""
+    G = T.CSRGraph.chung_lu(30000, 150000, 2.1, 3000, 21).with_labels(2, 5)
+    dg = dev.upload(G)
+    base = dev.tuning()
+    modes = [
+        dict(timeout_us=1, donate_levels=5),
+        dict(timeout_us=1, donate_levels=5, queue_log2=8),
+        dict(timeout_us=-1),
+        dict(chunk_edges=1),
+        dict(chunk_edges=1024, timeout_us=1),
+        dict(split_chunk=7, timeout_us=1, donate_levels=5),
+        dict(reuse=2),
+    ]
+    for i in (1, 2, 3, 5, 7, 11, 13):
+        for labeled in (0, 1):
+            for mode in modes:
+                say("sched", f"{i}.g", labeled, mode)
+                r = check(dev, dg, G, fixture(i), labeled, stats=mode.get("reuse") != 2, **mode)
+                dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_ if f != "reserved"})
+
+
+def test_hub_beyond_reference_buffer(dev):
+    """A hub of degree 6000 > 4096 (the reference's per-level slot size,
+    src/config.h:12): lists are never truncated here."""
+    n = 6500
+    edges = [(0, v) for v in range(1, 6001)]
+    rng = np.random.default_rng(3)
+    for _ in range(30000):
+        a, b = rng.integers(1, n, size=2)
+        edges.append((int(a), int(b)))
+    G = T.CSRGraph.from_edges(n, edges).with_labels(2, 8)
+    dg = dev.upload(G)
+    for text in [fixture(1), fixture(4), "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n",
+                 "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 4 1\n"]:
+        for labeled in (0, 1):
+            say("hub", text.count("v "), labeled)
+            check(dev, dg, G, text, labeled)
+
+
+def test_batch_equals_alone(dev):
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.1, 2000, 31).with_labels(3, 9)
+    dg = dev.upload(G)
+    rng = np.random.default_rng(5)
+    plans = []
+    for j in range(24):
+        k = int(rng.integers(3, 7))
+        m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+        text = T.gen_pattern(k, m, 3 if j % 2 else 0, int(rng.integers(1 << 62)))
+        plans.append(T.plan_pattern(text, j % 2))
+    dev.set_stats(True)
+    say("batch of", len(plans))
+    batch = dev.count_batch(dg, plans)
+    for j, p in enumerate(plans):
+        say("alone", j)
+        alone = dev.count(dg, p)
+        assert alone.raw == batch[j].raw, j
+        assert alone.alg_bytes == batch[j].alg_bytes, j
+
+
+def test_edge_ranges_partition(dev):
+    """Level-0 shards (directed-edge ranges) add up to the whole count."""
+    G = T.CSRGraph.chung_lu(50000, 250000, 2.1, 4000, 41)
+    dg = dev.upload(G)
+    for i in (1, 2, 6):
+        p = T.plan_pattern(fixture(i), 0)
+        whole = dev.count(dg, p).raw
+        cuts = [0, 12345, G.nnz // 2, G.nnz - 1, G.nnz]
+        say("ranges", f"{i}.g")
+        parts = [dev.count(dg, p, a, b).raw for a, b in zip(cuts, cuts[1:])]
+        assert sum(parts) == whole
+
+
+def test_config1_triangle_er(dev):
+    """ER G(n=20000, m=100000) seed 1, triangle: count == trace(A^3)."""
+    G = T.CSRGraph.erdos_renyi(20000, 100000, 1)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((np.ones(G.nnz, dtype=np.int64), G.colidx, G.rowptr), shape=(G.n, G.n))
+    tr = int((A @ A).multiply(A).sum())
+    dg = dev.upload(G)
+    say("config1 triangle")
+    r = check(dev, dg, G, "v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n", 0)
+    assert r.count == tr
+    assert r.raw * 6 == tr
+
+
+def test_empty_and_tiny(dev):
+    E = T.CSRGraph.from_edges(10, [])
+    dg = dev.upload(E)
+    say("empty graph")
+    assert dev.count(dg, T.plan_pattern(fixture(2), 0)).count == 0
+    K3 = T.CSRGraph.from_edges(3, [(0, 1), (1, 2), (0, 2)])
+    dg = dev.upload(K3)
+    assert dev.count(dg, T.plan_pattern(fixture(2), 0)).count == 0
+    assert dev.count(dg, T.plan_pattern("v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n", 0)).count == 6
+    assert dev.count(dg, T.plan_pattern(fixture(2), 0), 2, 2).count == 0
+
+
+def test_labeled_plan_needs_labels(dev):
+    G = T.CSRGraph.erdos_renyi(100, 300, 1)
+    dg = dev.upload(G)
+    with pytest.raises(T.TdfsError):
+        dev.count(dg, T.plan_pattern(fixture(8), 1))

@@ -1,0 +1,205 @@
+"""Host side of libtdfs (no GPU calls): the planner against the oracle's,
+the reference binary CSR format, seeded generators, the reorder step, error
+behaviour, and the exported C ABI."""
+import ctypes
+import hashlib
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import tdfs_amd as T
+from tdfs_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def same_plan(p, q):
+    k = p.k
+    assert k == q.k
+    assert p.multiplicity == q.mult
+    assert p.sym01 == q.sym01
+    assert p.npairs == q.npairs
+    assert [tuple(p.pairs[i]) for i in range(p.npairs)] == [tuple(q.pairs[i]) for i in range(q.npairs)]
+    for u in range(k):
+        assert p.deg[u] == q.deg[u]
+        assert p.label[u] == q.vlabel[u]
+        assert list(p.bn[u])[: p.num_bn[u]] == list(q.bn[u])[: q.num_bn[u]]
+        assert p.shared_lvl[u] == q.shared_lvl[u]
+        if p.shared_lvl[u] >= 0:
+            assert list(p.bn_sh[u])[: p.num_bn_sh[u]] == list(q.bn_sh[u])[: q.num_bn_sh[u]]
+        for j in range(u):
+            assert p.cond[u][j] == q.cond_op[u][j]
+
+
+@pytest.mark.parametrize("labeled", [0, 1])
+def test_planner_matches_oracle_fixtures(labeled):
+    for i in range(1, 15):
+        text = (GOLDEN / "patterns" / f"{i}.g").read_text()
+        same_plan(T.plan_pattern(text, labeled), O.plan(text, labeled))
+
+
+def test_planner_matches_oracle_random():
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        k = int(rng.integers(3, 8))
+        m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+        text = T.gen_pattern(k, m, int(rng.integers(0, 4)), int(rng.integers(1 << 62)))
+        for labeled in (0, 1):
+            same_plan(T.plan_pattern(text, labeled), O.plan(text, labeled))
+
+
+def test_pattern_errors():
+    with pytest.raises(T.TdfsError):
+        T.plan_pattern("v 0 1\nv 1 1\ne 0 1 1\n", 0)  # k < 3
+    with pytest.raises(T.TdfsError):
+        T.plan_pattern("v 0 1\nv 1 1\nv 2 1\ne 0 1 2\ne 1 2 1\n", 0)  # edge label != 1
+    with pytest.raises(T.TdfsError):
+        T.plan_pattern("v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 1 2 1\n", 0)  # v3 has no BN
+    with pytest.raises(T.TdfsError):
+        T.load_pattern("/nonexistent.g", 0)
+
+
+def test_reorder_properties():
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        k = int(rng.integers(3, 8))
+        m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+        text = T.gen_pattern(k, m, 3, int(rng.integers(1 << 62)))
+        p = T.plan_pattern(text, 1)  # generated patterns are already ordered
+        assert p.adj[2][1] == 1
+        for u in range(1, k):
+            assert p.num_bn[u] >= 1
+    # reordering a badly ordered pattern keeps its shape and labels
+    bad = "v 0 5\nv 1 6\nv 2 7\nv 3 8\nv 4 9\ne 0 1 1\ne 0 2 1\ne 2 3 1\ne 2 4 1\n"
+    out, order = T.reorder_pattern(bad)
+    p = T.plan_pattern(out, 1)
+    assert p.adj[2][1] == 1
+    assert sorted(order) == list(range(5))
+    assert [p.label[i] for i in range(5)] == [5 + order[i] for i in range(5)]
+
+
+def test_reorder_preserves_counts():
+    G = T.CSRGraph.erdos_renyi(11, 30, 2).with_labels(2, 1)
+    rng = np.random.default_rng(4)
+    for _ in range(25):
+        k = int(rng.integers(3, 6))
+        m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+        text = T.gen_pattern(k, m, 2, int(rng.integers(1 << 62)))
+        # scramble the vertex order, then reorder
+        perm = rng.permutation(k)
+        lines = text.splitlines()
+        vs = [l for l in lines if l.startswith("v")]
+        es = [l for l in lines if l.startswith("e")]
+        inv = np.argsort(perm)
+        labs = {int(l.split()[1]): l.split()[2] for l in vs}
+        scr = "".join(f"v {i} {labs[int(inv[i])]}\n" for i in range(k))
+        for l in es:
+            _, a, b, _x = l.split()
+            scr += f"e {perm[int(a)]} {perm[int(b)]} 1\n"
+        back, _ = T.reorder_pattern(scr)
+        for labeled in (0, 1):
+            want = O.bruteforce(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(text, labeled))
+            got = O.count(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(back, labeled))["count"]
+            assert got == want
+
+
+def test_bin_format_roundtrip(tmp_path):
+    G = T.CSRGraph.chung_lu(500, 2000, 2.2, 60, 3).with_labels(4, 9)
+    prefix = str(tmp_path / "g")
+    G.write_bin(prefix)
+    meta = (tmp_path / "g.meta.txt").read_text().split()
+    assert meta == [str(G.n), str(G.nnz), "4", str(G.dmax)]
+    # on-disk types: uint64 offsets, uint32 ids and labels, little-endian
+    assert np.array_equal(np.fromfile(tmp_path / "g.vertex.bin", dtype="<u8"), G.rowptr.astype(np.uint64))
+    assert np.array_equal(np.fromfile(tmp_path / "g.edge.bin", dtype="<u4"), G.colidx.astype(np.uint32))
+    assert np.array_equal(np.fromfile(tmp_path / "g.label.bin", dtype="<u4"), G.labels.astype(np.uint32))
+    H = T.CSRGraph.read_bin(prefix, labeled=True)
+    assert np.array_equal(H.rowptr, G.rowptr) and np.array_equal(H.colidx, G.colidx)
+    assert np.array_equal(H.labels, G.labels)
+    U = T.CSRGraph.read_bin(prefix, labeled=False)
+    assert U.labels is None
+
+
+def test_bin_format_errors(tmp_path):
+    with pytest.raises(T.TdfsError):
+        T.CSRGraph.read_bin(str(tmp_path / "missing"))
+    (tmp_path / "x.meta.txt").write_text("3 4 8 2\n")  # vid_size 8 is rejected (src/graph.h:145)
+    with pytest.raises(T.TdfsError):
+        T.CSRGraph.read_bin(str(tmp_path / "x"))
+
+
+def test_validate_rejects_bad_csr():
+    good = T.CSRGraph.from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    good.validate()
+    cases = [
+        (np.array([0, 1, 2], np.int64), np.array([0, 0], np.int32)),          # self-loops
+        (np.array([0, 2, 3, 4], np.int64), np.array([2, 1, 0, 0], np.int32)),  # unsorted
+        (np.array([0, 1, 1], np.int64), np.array([1], np.int32)),             # asymmetric
+        (np.array([0, 2, 3], np.int64), np.array([1, 1, 0], np.int32)),       # duplicate
+        (np.array([0, 1, 2], np.int64), np.array([5, 0], np.int32)),          # out of range
+    ]
+    for rp, ci in cases:
+        with pytest.raises(T.TdfsError):
+            T.CSRGraph(rp, ci).validate()
+
+
+def _sha(g):
+    h = hashlib.sha256()
+    h.update(g.rowptr.tobytes())
+    h.update(g.colidx.tobytes())
+    if g.labels is not None:
+        h.update(g.labels.tobytes())
+    return h.hexdigest()[:16]
+
+
+def test_generators_deterministic_and_valid():
+    a = T.CSRGraph.erdos_renyi(2000, 10000, 1)
+    b = T.CSRGraph.erdos_renyi(2000, 10000, 1)
+    assert _sha(a) == _sha(b)
+    a.validate()
+    assert a.nnz == 20000
+    c = T.CSRGraph.chung_lu(20000, 60000, 2.1, 600, 2)
+    c.validate()
+    assert c.nnz == 120000
+    # heavy tail: the top vertex is near the requested maximum expected degree
+    assert 300 < c.dmax < 1200
+    assert _sha(c) == _sha(T.CSRGraph.chung_lu(20000, 60000, 2.1, 600, 2))
+    assert _sha(c) != _sha(T.CSRGraph.chung_lu(20000, 60000, 2.1, 600, 3))
+    lab = c.with_labels(4, 3)
+    assert set(np.unique(lab.labels)) == {0, 1, 2, 3}
+
+
+def test_generator_golden_hashes():
+    """Pins the generators: a change in the PRNG or the sampling rule changes
+    every benchmark input."""
+    hashes = {
+        "er_20000_100000_s1": _sha(T.CSRGraph.erdos_renyi(20000, 100000, 1)),
+        "cl_50000_150000_s2": _sha(T.CSRGraph.chung_lu(50000, 150000, 2.1, 1500, 2).with_labels(4, 3)),
+    }
+    path = GOLDEN / "generator_hashes.json"
+    import json
+    want = json.loads(path.read_text())
+    assert hashes == want
+
+
+def test_c_abi_exports_every_declared_symbol():
+    header = (ROOT / "include" / "tdfs.h").read_text()
+    decl = set(re.findall(r"\b(tdfs_[a-z_0-9]+)\s*\(", header))
+    assert len(decl) > 20
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [s for s in sorted(decl) if not hasattr(lib, s)]
+    assert not missing, missing
+    # the ctypes shim binds exactly the declared functions
+    assert set(_lib._SIGS) <= decl
+
+
+def test_cli_usage_without_gpu():
+    for name in ("ulb", "lb"):
+        exe = ROOT / "tdfs_amd" / "bin" / name
+        r = subprocess.run([str(exe)], capture_output=True, text=True)
+        assert r.returncode == 2 and "usage" in r.stderr
