@@ -52,6 +52,9 @@ def lib():
         L.orc_count.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32), C.POINTER(OrcPlan), C.c_int64, C.c_int64, C.c_int, C.c_int,
                                 C.POINTER(C.c_uint64)]
+        L.orc_count_ranges.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(OrcPlan), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
         L.orc_bruteforce_count.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                            C.POINTER(C.c_int32), C.POINTER(OrcPlan)]
         L.orc_bruteforce_count.restype = C.c_uint64
@@ -100,6 +103,24 @@ def count(rowptr, colidx, labels, p: OrcPlan, edge_lo: int = 0, edge_hi: int = -
         "extensions": int(sum(out[2:9])),
         "nodes": [int(x) for x in out[2:9]],
     }
+
+
+def count_ranges(rowptr, colidx, labels, p: OrcPlan, los, his, threads: int = 1, reuse: bool = True) -> dict:
+    """As count(), over the union of edge windows [los[i], his[i])."""
+    rp, ci, lb = _ptrs(rowptr, colidx, labels)
+    lo = np.ascontiguousarray(los, dtype=np.int64)
+    hi = np.ascontiguousarray(his, dtype=np.int64)
+    out = (C.c_uint64 * 9)()
+    rc = lib().orc_count_ranges(rp.shape[0] - 1, ci.shape[0], rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                ci.ctypes.data_as(C.POINTER(C.c_int32)),
+                                None if lb is None else lb.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(p),
+                                lo.ctypes.data_as(C.POINTER(C.c_int64)), hi.ctypes.data_as(C.POINTER(C.c_int64)),
+                                lo.shape[0], threads, int(reuse), out)
+    if rc:
+        raise ValueError(f"oracle count failed ({rc})")
+    raw = int(out[0])
+    return {"raw": raw, "count": raw if p.labeled else raw * int(p.mult), "alg_bytes": int(out[1]),
+            "extensions": int(sum(out[2:9]))}
 
 
 def bruteforce(rowptr, colidx, labels, p: OrcPlan) -> int:

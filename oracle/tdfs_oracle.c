@@ -427,11 +427,24 @@ static int edge_pre_ok(const orc_worker *w, int64_t e) { /* order/label part onl
 
 typedef struct {
   orc_worker w;
-  int64_t lo, hi, chunk;
+  const int64_t *lo, *hi, *pre; /* ranges and prefix sums of their lengths */
+  int nranges;
+  int64_t total, chunk;
   int64_t *cursor;
   pthread_mutex_t *mu;
   pthread_t th;
 } orc_thread;
+
+static void run_edge(orc_worker *w, int64_t e) {
+  w->balg += 8; /* colidx + row id */
+  if (!edge_pre_ok(w, e)) return;
+  w->balg += 16; /* rowptr pair reads of the degree prune */
+  if (!edge_ok(w, e)) return;
+  w->nodes[1]++;
+  w->M[0] = w->g->src[e];
+  w->M[1] = w->g->colidx[e];
+  dfs(w, 2);
+}
 
 static void *thread_main(void *arg) {
   orc_thread *t = (orc_thread *)arg;
@@ -442,38 +455,47 @@ static void *thread_main(void *arg) {
     s = *t->cursor;
     *t->cursor += t->chunk;
     pthread_mutex_unlock(t->mu);
-    if (s >= t->hi) break;
-    int64_t e1 = s + t->chunk < t->hi ? s + t->chunk : t->hi;
-    for (int64_t e = s; e < e1; e++) {
-      w->balg += 8; /* colidx + row id */
-      if (!edge_pre_ok(w, e)) continue;
-      w->balg += 16; /* rowptr pair reads of the degree prune */
-      if (!edge_ok(w, e)) continue;
-      w->nodes[1]++;
-      w->M[0] = w->g->src[e];
-      w->M[1] = w->g->colidx[e];
-      dfs(w, 2);
+    if (s >= t->total) break;
+    int64_t s1 = s + t->chunk < t->total ? s + t->chunk : t->total;
+    /* virtual positions [s, s1) -> (range, offset) */
+    int r = 0;
+    while (r + 1 < t->nranges && t->pre[r + 1] <= s) r++;
+    for (int64_t v = s; v < s1; v++) {
+      while (v >= t->pre[r + 1]) r++;
+      run_edge(w, t->lo[r] + (v - t->pre[r]));
     }
   }
   return NULL;
 }
 
-/* Counts over directed edges [edge_lo, edge_hi) of the CSR.
+/* Counts over the union of directed-edge ranges [lo[i], hi[i]) of the CSR
+ * (ranges clipped to [0, nnz); overlapping ranges are counted twice).
  * out[0] = raw count (pre-multiplicity, cu_test.cu:103)
  * out[1] = algorithmic bytes
  * out[2+u], u = 0..6: extensions computing C(u) (u = 1: accepted level-0 edges)
  * Returns 0, or a negative code. */
-int orc_count(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
-              const int32_t *labels, const orc_plan *p, int64_t edge_lo, int64_t edge_hi,
-              int nthreads, int reuse, uint64_t *out) {
+int orc_count_ranges(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+                     const int32_t *labels, const orc_plan *p, const int64_t *lo_in,
+                     const int64_t *hi_in, int nranges, int nthreads, int reuse, uint64_t *out) {
   if (p->labeled && !labels) return -1;
+  if (nranges < 1) return -2;
   orc_graph g = {n, nnz, rowptr, colidx, labels, NULL, NULL, 0};
   graph_prepare(&g);
-  if (edge_lo < 0) edge_lo = 0;
-  if (edge_hi > nnz || edge_hi < 0) edge_hi = nnz;
+  int64_t *lo = (int64_t *)malloc(sizeof(int64_t) * (size_t)nranges);
+  int64_t *hi = (int64_t *)malloc(sizeof(int64_t) * (size_t)nranges);
+  int64_t *pre = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nranges + 1));
+  pre[0] = 0;
+  for (int i = 0; i < nranges; i++) {
+    int64_t a = lo_in[i] < 0 ? 0 : lo_in[i];
+    int64_t b = (hi_in[i] < 0 || hi_in[i] > nnz) ? nnz : hi_in[i];
+    if (a > b) a = b;
+    lo[i] = a;
+    hi[i] = b;
+    pre[i + 1] = pre[i] + (b - a);
+  }
   if (nthreads < 1) nthreads = 1;
   orc_thread *ts = (orc_thread *)calloc((size_t)nthreads, sizeof(orc_thread));
-  int64_t cursor = edge_lo;
+  int64_t cursor = 0;
   pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
   for (int i = 0; i < nthreads; i++) {
     orc_thread *t = &ts[i];
@@ -482,8 +504,11 @@ int orc_count(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *coli
     t->w.reuse = reuse;
     for (int u = 0; u < ORC_MAXK; u++)
       t->w.cand[u] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(g.dmax > 0 ? g.dmax : 1));
-    t->lo = edge_lo;
-    t->hi = edge_hi;
+    t->lo = lo;
+    t->hi = hi;
+    t->pre = pre;
+    t->nranges = nranges;
+    t->total = pre[nranges];
     t->chunk = 64;
     t->cursor = &cursor;
     t->mu = &mu;
@@ -500,8 +525,18 @@ int orc_count(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *coli
     for (int u = 0; u < ORC_MAXK; u++) free(t->w.cand[u]);
   }
   free(ts);
+  free(lo);
+  free(hi);
+  free(pre);
   graph_release(&g);
   return 0;
+}
+
+int orc_count(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+              const int32_t *labels, const orc_plan *p, int64_t edge_lo, int64_t edge_hi,
+              int nthreads, int reuse, uint64_t *out) {
+  return orc_count_ranges(n, nnz, rowptr, colidx, labels, p, &edge_lo, &edge_hi, 1, nthreads, reuse,
+                          out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -147,18 +147,19 @@ template <typename T>
 __device__ __forceinline__ void st_rel(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Counters are relaxed: ordering of the ring payload is carried only by the
+// per-slot sequence numbers (release on publish, acquire on take).  A
+// seq_cst or acq_rel RMW at agent scope would write back and invalidate the
+// XCD's L2 on every call.
 template <typename T>
 __device__ __forceinline__ T fadd(T *p, T v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T ld_rlx(T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <typename T>
-__device__ __forceinline__ T ld_sc(T *p) {
-  return __hip_atomic_load(p, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-}
+
 
 // First index i in sorted a[0, n) with a[i] >= x (uniform or per-lane).
 __device__ __forceinline__ int lower_bound_i(const int *a, int n, int x) {
@@ -425,7 +426,7 @@ __device__ bool ring_take(const KParams &P, TaskRec &t) {
         }
       }
     }
-    if (err) __hip_atomic_store(&C->error, 2, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    if (err) __hip_atomic_store(&C->error, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   got = uni(got);
   if (got) {
@@ -441,10 +442,11 @@ __device__ bool ring_take(const KParams &P, TaskRec &t) {
 }
 
 // Donates the untaken iterations of the shallowest open level in
-// [floor, level].  Returns the number of tasks pushed (0 when the ring is
-// full).  All lanes call it.
+// [floor, level], cut into about 2*idle pieces (at least split_chunk
+// iterations each, at most 256 pieces).  Returns the number of tasks pushed
+// (0 when the ring is full).  All lanes call it.
 __device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
-                      const int *L0c) {
+                      const int *L0c, int idle) {
   int ld = level;
   for (int l = floor; l <= level; l++) {
     if (S.it[l] < S.end[l]) {
@@ -457,8 +459,11 @@ __device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int leve
   if (rem <= 0) return 0;
   int piece = 1, ntasks = rem;
   if (ld > 0) {
-    piece = max(P.split_chunk, (rem + 255) / 256);
+    const int want = max(1, min(256, 2 * idle));
+    piece = max(P.split_chunk, (rem + want - 1) / want);
     ntasks = (rem + piece - 1) / piece;
+  } else {
+    ntasks = min(rem, 256);
   }
   const unsigned long long pos0 = ring_reserve(P, ntasks);
   if (pos0 == ~0ull) return 0;
@@ -471,8 +476,8 @@ __device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int leve
     for (int q = 0; q < 8; q++) t.M[q] = 0;
     if (ld == 0) {
       t.p = 2;
-      t.M[0] = L0r[it + j];
-      t.M[1] = L0c[it + j];
+      t.M[0] = L0r[end - ntasks + j];
+      t.M[1] = L0c[end - ntasks + j];
       t.a = 0;
       t.b = INT_MAX;
       t.fresh = 1;
@@ -489,10 +494,12 @@ __device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int leve
   }
   if (__ballot(!okall) != 0ull) {
     if (lane_id() == 0)
-      __hip_atomic_store(&P.ctrl->error, 3, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&P.ctrl->error, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (lane_id() == 0) fadd(&P.ctrl->q_avail, (long long)ntasks);
-  S.end[ld] = it;  // every lane stores the same value
+  // every lane stores the same value; level 0 hands over its last ntasks edges
+  if (ld > 0) S.end[ld] = it;
+  else S.end[0] = end - ntasks;
   return ntasks;
 }
 
@@ -519,11 +526,16 @@ __device__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S
       const long long now = wall_clock64();
       if (now - t0 > P.timeout_ticks) {
         t0 = now;
+        // donate only while idle waves outnumber the tasks already queued
         int idle = 0;
-        if (lane_id() == 0) idle = ld_rlx(&P.ctrl->idle);
+        if (lane_id() == 0) {
+          idle = ld_rlx(&P.ctrl->idle);
+          const long long queued = ld_rlx(&P.ctrl->q_avail);
+          if ((long long)idle <= queued) idle = 0;
+        }
         idle = uni(idle);
         if (idle > 0) {
-          const int n = donate(P, S, pid, floor, level, L0r, L0c);
+          const int n = donate(P, S, pid, floor, level, L0r, L0c, idle);
           don += (unsigned)n;
           if (n > 0) continue;  // re-read this level: its end may have moved
         }
@@ -600,10 +612,10 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           int stop = 0;
           if (lane == 0) {
             if (ld_rlx(&C->error)) stop = 1;
-            else if (ld_sc(&C->q_avail) <= 0 && ld_sc(&C->busy) == 0 && ld_sc(&C->q_avail) <= 0)
+            else if (ld_rlx(&C->q_avail) <= 0 && ld_rlx(&C->busy) == 0 && ld_rlx(&C->q_avail) <= 0)
               stop = 1;
             else if (wall_clock64() - t_idle > P.deadline_ticks) {
-              __hip_atomic_store(&C->error, 4, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_store(&C->error, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               stop = 1;
             }
           }
@@ -755,7 +767,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->waves_per_block = kWavesPerBlock;
   t->blocks_per_cu = 0;
   t->chunk_edges = 64;
-  t->timeout_us = 20;
+  t->timeout_us = 200;
   t->split_chunk = 1;
   t->queue_log2 = 18;
   t->donate_levels = 3;

