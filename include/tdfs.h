@@ -83,7 +83,10 @@ typedef struct tdfs_tuning {
   int32_t donate_levels;     /* donate only from stack levels < this (default 3) */
   int32_t reuse;             /* 1: result reuse (default); 2: off (testing) */
   int32_t lds_list_max;      /* reserved */
-  int32_t reserved[7];
+  int32_t leaf_batch;        /* 1: count the leaf for 64 parents at once (default); 2: one at a time */
+  int32_t hub_min_degree;    /* vertices of at least this degree get a neighbour bitmap (default 256) */
+  int32_t hub_budget_mb;     /* bitmap memory per uploaded graph (default 4096; <0: none); read at upload */
+  int32_t reserved[4];
 } tdfs_tuning;
 
 /* Result of one counted pattern. */
@@ -151,6 +154,11 @@ int tdfs_ctx_get_tuning(const tdfs_ctx *ctx, tdfs_tuning *t);
 /* stats 1: every launch also accumulates alg_bytes / extensions */
 int tdfs_ctx_set_stats(tdfs_ctx *ctx, int32_t on);
 void tdfs_ctx_destroy(tdfs_ctx *ctx);
+/* Cycle counters of the last launch made with stats on (shader clock,
+ * summed over waves): out8 = {level-0 filter, non-leaf extensions, leaf
+ * counts, idle waiting, prefix rebuild of received tasks, donation, total,
+ * number of waves}. */
+int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8);
 /* device properties: out4 = {compute units, wall-clock kHz, max waves resident, reserved} */
 int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4);
 

@@ -161,6 +161,13 @@ class Device:
         check(lib().tdfs_ctx_info(self.handle, i64p(out)), "ctx info")
         return {"cus": int(out[0]), "wallclock_khz": int(out[1]), "waves": int(out[2]), "blocks_per_cu": int(out[3])}
 
+    def last_profile(self) -> dict:
+        """Shader-clock cycles of the last stats launch, summed over waves."""
+        out = (C.c_uint64 * 8)()
+        check(lib().tdfs_ctx_last_profile(self.handle, out), "last profile")
+        keys = ["filter", "extend", "leaf", "idle", "rebuild", "donate", "total", "waves"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def tuning(self) -> Tuning:
         t = Tuning()
         check(lib().tdfs_ctx_get_tuning(self.handle, C.byref(t)), "get tuning")

@@ -45,7 +45,10 @@ class Tuning(C.Structure):
         ("donate_levels", C.c_int32),
         ("reuse", C.c_int32),
         ("lds_list_max", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("leaf_batch", C.c_int32),
+        ("hub_min_degree", C.c_int32),
+        ("hub_budget_mb", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -84,6 +87,7 @@ _SIGS = {
     "tdfs_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tdfs_ctx_set_stats": (C.c_int, [_P, C.c_int32]),
     "tdfs_ctx_info": (C.c_int, [_P, _I64P]),
+    "tdfs_ctx_last_profile": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "tdfs_ctx_destroy": (None, [_P]),
     "tdfs_graph_upload": (C.c_int, [_P, C.c_int32, C.c_int64, _I64P, _I32P, _I32P, C.POINTER(_P)]),
     "tdfs_graph_free": (None, [_P]),

@@ -42,6 +42,7 @@ constexpr int kChunkMax = 1024;      // level-0 edges per cursor grab, upper bou
 constexpr int kWavesPerBlock = 4;    // 256-thread workgroups
 constexpr int kBlock = kWavesPerBlock * kWave;
 constexpr long long kSpinMax = 1ll << 24;  // bound on waits for a slot handshake
+constexpr int kLeafPairCap = 4096;         // (owner, candidate) pairs per leaf batch
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
 struct DevPlan {
@@ -94,6 +95,9 @@ struct KParams {
   const int *src;
   const int *maxnbr;
   const unsigned char *labels;
+  const int *hub_idx;         // per vertex: row of its neighbour bitmap, or -1
+  const unsigned *hub_bits;   // hub_words 32-bit words per hub
+  long long hub_words;
   // plans and the level-0 cursor space
   const DevPlan *plans;
   int nplans;
@@ -112,6 +116,7 @@ struct KParams {
   // results: counts[pid]; stats[pid*4 + {0 alg bytes, 1 extensions, 2 donated}]
   unsigned long long *counts;
   unsigned long long *stats;
+  unsigned long long *prof;  // 8 per launch: cycles in filter, extend, leaf, idle, rebuild, donate, total; waves
   Ctrl *ctrl;
   // tuning
   long long timeout_ticks;  // < 0: never donate
@@ -119,6 +124,7 @@ struct KParams {
   int split_chunk;
   int donate_levels;
   int reuse;
+  int leaf_batch;
 };
 
 struct WaveLds {
@@ -189,21 +195,31 @@ __device__ __forceinline__ bool contains(const int *a, int n, int x) {
   return *base == x;
 }
 
+// Membership through a hub's dense neighbour bitmap: one load.
+__device__ __forceinline__ bool bit_test(const unsigned *bits, int x) { return (bits[x >> 5] >> (x & 31)) & 1u; }
+
+__device__ __forceinline__ const unsigned *hub_row(const KParams &P, int v) {
+  const int h = P.hub_idx[v];
+  return h >= 0 ? P.hub_bits + (long long)h * P.hub_words : nullptr;
+}
+
 // ---------------------------------------------------------------------------
 // Candidate generation for pattern vertex u (2 <= u <= k-1) given M(0..u-1).
 // Returns |C(u)|; writes C(u) (ascending) to out unless out == nullptr.
 // ---------------------------------------------------------------------------
 template <bool STATS>
-__device__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int u,
+__device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int u,
                            int *__restrict__ out, const int *lbase, bool count_stats,
                            unsigned long long &balg, unsigned long long &next) {
   const int lane = lane_id();
   const int dep = P.reuse ? pl->dep[u] : -1;
   const int *lp[7];
+  const unsigned *hb[7];
   int ln[7];
 #pragma unroll
   for (int i = 0; i < 7; i++) {
     lp[i] = nullptr;
+    hb[i] = nullptr;
     ln[i] = 0;
   }
   int nl = 0;
@@ -219,6 +235,7 @@ __device__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, con
         const int t = S.M[pl->bn_sh[u][i]];
         lp[i + 1] = P.colidx + P.rowptr[t];
         ln[i + 1] = P.degv[t];
+        hb[i + 1] = hub_row(P, t);
         bytes += 4ull * (unsigned)ln[i + 1] + 16;
       }
     }
@@ -231,6 +248,7 @@ __device__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, con
         const int t = S.M[pl->bn[u][i]];
         lp[i] = P.colidx + P.rowptr[t];
         ln[i] = P.degv[t];
+        hb[i] = hub_row(P, t);
         bytes += 4ull * (unsigned)ln[i] + 16;
       }
     }
@@ -304,7 +322,7 @@ __device__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, con
           if (labeled && ok) ok = (int)P.labels[x] == lab;
 #pragma unroll
           for (int i = 0; i < 7; i++)
-            if (i < nl && i != pidx && ok) ok = contains(lp[i], ln[i], x);
+            if (i < nl && i != pidx && ok) ok = hb[i] ? bit_test(hb[i], x) : contains(lp[i], ln[i], x);
         }
         const unsigned long long mask = __ballot(ok);
         if (out) {
@@ -323,10 +341,215 @@ __device__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, con
   return n;
 }
 
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// Count-only leaf for elements [i0, i1) (at most 64) of stack level L = k-3:
+// element x fixes M(k-2) (for L = 0 the edge pair fixes M(0), M(1)) and
+// contributes |C(k-1)|.  Lane l owns element i0 + l and builds that
+// element's lists, window and probe; the probe candidates of all owners are
+// then flattened into one pair space (wave prefix scan) and each lane tests
+// one (owner, candidate) pair per step, finding its owner by a binary search
+// over the scan with ds_bpermute.  Same result and statistics as calling
+// extend() for each element in turn.
+template <bool STATS>
+__device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int L,
+                                         int i0, int i1, const int *L0r, const int *L0c, const int *lbase,
+                                         unsigned long long &balg, unsigned long long &next, int &taken) {
+  const int lane = lane_id();
+  const int k = pl->k;
+  const int u = k - 1;
+  const bool labeled = pl->labeled != 0;
+  const int idx = i0 + lane;
+  const bool act = idx < i1;
+  int pm[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) pm[j] = S.M[j];
+  if (L == 0) {
+    pm[0] = act ? L0r[idx] : 0;
+    pm[1] = act ? L0c[idx] : 0;
+  } else {
+    const int x = act ? lbase[(long long)(L - 1) * P.dstride + idx] : 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+      if (j == L + 1) pm[j] = x;
+  }
+  auto pick = [&](int b) {
+    int t = 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+      if (b == j) t = pm[j];
+    return t;
+  };
+  const int dep = P.reuse ? pl->dep[u] : -1;
+  const int *lp[7];
+  const unsigned *hb[7];
+  int ln[7];
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    lp[i] = nullptr;
+    hb[i] = nullptr;
+    ln[i] = 0;
+  }
+  int nl = 0;
+  unsigned long long bytes = 0;
+  if (dep >= 0) {
+    lp[0] = lbase + (long long)(dep - 2) * P.dstride;
+    ln[0] = S.len[dep - 1];
+    bytes += 4ull * (unsigned)ln[0];
+    const int nsh = pl->num_bn_sh[u];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      if (i < nsh) {
+        const int t = pick(pl->bn_sh[u][i]);
+        lp[i + 1] = P.colidx + P.rowptr[t];
+        ln[i + 1] = P.degv[t];
+        hb[i + 1] = hub_row(P, t);
+        bytes += 4ull * (unsigned)ln[i + 1] + 16;
+      }
+    }
+    nl = 1 + nsh;
+  } else {
+    const int nb = pl->num_bn[u];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      if (i < nb) {
+        const int t = pick(pl->bn[u][i]);
+        lp[i] = P.colidx + P.rowptr[t];
+        ln[i] = P.degv[t];
+        hb[i] = hub_row(P, t);
+        bytes += 4ull * (unsigned)ln[i] + 16;
+      }
+    }
+    nl = nb;
+  }
+  const int *pp = lp[0];
+  int pn = ln[0];
+  int pidx = 0;
+#pragma unroll
+  for (int i = 1; i < 7; i++) {
+    if (i < nl && ln[i] < pn) {
+      pp = lp[i];
+      pn = ln[i];
+      pidx = i;
+    }
+  }
+  if (STATS && labeled) bytes += (unsigned)pn;
+  int lo = -1, hi = INT_MAX;
+  int nev[6];
+  int nemask = 0;
+  const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    nev[j] = -1;
+    if (j < u) {
+      const int m = pm[j];
+      if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
+      else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
+      else {
+        nev[j] = m;
+        nemask |= 1 << j;
+      }
+    }
+  }
+  int start = 0, stop = 0;
+  if (act) {
+    stop = pn;
+    if (lo >= 0) start = lower_bound_i(pp, pn, lo + 1);
+    if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
+    if (stop < start) stop = start;
+  }
+  if (!labeled && nl == 1) {
+    // |list ∩ (lo,hi)| minus excluded path vertices inside the window
+    taken = i1 - i0;
+    if (STATS) {
+      balg += wave_sum(act ? bytes : 0ull);
+      next += (unsigned)__popcll(__ballot(act));
+    }
+    unsigned c = (unsigned)(stop - start);
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        if ((nemask >> j) & 1) {
+          const int v = nev[j];
+          if (v > lo && v < hi && stop > start) {
+            const int pos = lower_bound_i(pp + start, stop - start, v);
+            if (pos < stop - start && pp[start + pos] == v) c--;
+          }
+        }
+      }
+    }
+    return wave_sum(act ? (unsigned long long)c : 0ull);
+  }
+  int s = stop - start;
+  int incl = s;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int t = __shfl_up(incl, d);
+    if (lane >= d) incl += t;
+  }
+  // Take the longest prefix of owners whose pairs fit in kLeafPairCap (at
+  // least one owner), so that a batch stays short enough for the donation
+  // check between batches to keep the tail balanced.
+  int tk = __popcll(__ballot(act && incl <= kLeafPairCap));
+  if (tk == 0) tk = 1;
+  taken = tk;
+  const bool mine = lane < tk;
+  if (!mine) s = 0;
+  const int T = __shfl(incl, tk - 1);
+  if (STATS) {
+    balg += wave_sum(mine ? bytes : 0ull);
+    next += (unsigned)tk;
+  }
+  const int lab = pl->label[u];
+  unsigned long long found = 0;
+  for (int q0 = 0; q0 < T; q0 += kWave) {
+    const int q = q0 + lane;
+    const bool v = q < T;
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+      const int w = __shfl(incl, o + step - 1);
+      if (w <= q) o += step;
+    }
+    if (!v) o = 0;
+    const int excl_o = __shfl(incl, o) - __shfl(s, o);
+    const int *ppo = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(pp), o));
+    const int st = __shfl(start, o);
+    const int po = __shfl(pidx, o);
+    bool ok = v;
+    int y = 0;
+    if (ok) y = ppo[st + (q - excl_o)];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      if ((nemask >> j) & 1) {
+        const int ne = __shfl(nev[j], o);
+        ok = ok && (y != ne);
+      }
+    }
+    if (labeled && ok) ok = (int)P.labels[y] == lab;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      if (i < nl) {
+        const int *lpo = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(lp[i]), o));
+        const unsigned *hbo =
+            reinterpret_cast<const unsigned *>(__shfl(reinterpret_cast<unsigned long long>(hb[i]), o));
+        const int lno = __shfl(ln[i], o);
+        if (i != po && ok) ok = hbo ? bit_test(hbo, y) : contains(lpo, lno, y);
+      }
+    }
+    found += (unsigned)__popcll(__ballot(ok));
+  }
+  return found;
+}
+
 // Level-0 filter over directed edges [e0, e1) (src/gpu_match.cu:526-558).
 // "exists v in N(c) with deg(v) >= dP(2)" is maxnbr[c] >= dP(2).
 template <bool STATS>
-__device__ int filter_edges(const KParams &P, const DevPlan *__restrict__ pl, long long e0, long long e1,
+__device__ __forceinline__ int filter_edges(const KParams &P, const DevPlan *__restrict__ pl, long long e0, long long e1,
                             int *L0r, int *L0c, unsigned long long &balg) {
   const int lane = lane_id();
   const bool labeled = pl->labeled != 0;
@@ -369,7 +592,7 @@ __device__ int filter_edges(const KParams &P, const DevPlan *__restrict__ pl, lo
 
 // Reserves n slots; returns the first position or ~0ull when full.  Lane 0
 // performs the accounting; the result is wave-uniform.
-__device__ unsigned long long ring_reserve(const KParams &P, int n) {
+__device__ __forceinline__ unsigned long long ring_reserve(const KParams &P, int n) {
   unsigned long long pos = ~0ull;
   if (lane_id() == 0) {
     Ctrl *C = P.ctrl;
@@ -385,7 +608,7 @@ __device__ unsigned long long ring_reserve(const KParams &P, int n) {
 }
 
 // Lane-parallel publication of one slot (called by the lane that owns it).
-__device__ bool ring_publish(const KParams &P, unsigned long long pos, const TaskRec &t) {
+__device__ __forceinline__ bool ring_publish(const KParams &P, unsigned long long pos, const TaskRec &t) {
   unsigned long long *seq = P.q_seq + (pos & P.q_mask);
   long long spins = 0;
   while (ld_acq(seq) != pos) {
@@ -398,7 +621,7 @@ __device__ bool ring_publish(const KParams &P, unsigned long long pos, const Tas
 }
 
 // Claims and reads one task.  Wave-uniform result.
-__device__ bool ring_take(const KParams &P, TaskRec &t) {
+__device__ __forceinline__ bool ring_take(const KParams &P, TaskRec &t) {
   Ctrl *C = P.ctrl;
   int got = 0;
   int err = 0;
@@ -445,7 +668,7 @@ __device__ bool ring_take(const KParams &P, TaskRec &t) {
 // [floor, level], cut into about 2*idle pieces (at least split_chunk
 // iterations each, at most 256 pieces).  Returns the number of tasks pushed
 // (0 when the ring is full).  All lanes call it.
-__device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
+__device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
                       const int *L0c, int idle) {
   int ld = level;
   for (int l = floor; l <= level; l++) {
@@ -507,9 +730,10 @@ __device__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int leve
 // set).  Stack level L iterates C(L+1) (L >= 1) or the edge pairs (L = 0);
 // taking element x at level L fixes M(L+1) = x and computes C(L+2).
 template <bool STATS>
-__device__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
+__device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
                     const int *L0r, const int *L0c, int *lbase, unsigned long long &cnt,
-                    unsigned long long &balg, unsigned long long &next, unsigned long long &don) {
+                    unsigned long long &balg, unsigned long long &next, unsigned long long &don,
+                    long long *prof) {
   const int k = pl->k;
   int level = floor;
   long long t0 = wall_clock64();
@@ -535,11 +759,21 @@ __device__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S
         }
         idle = uni(idle);
         if (idle > 0) {
+          const long long tp = STATS ? clock64() : 0;
           const int n = donate(P, S, pid, floor, level, L0r, L0c, idle);
+          if (STATS) prof[5] += clock64() - tp;
           don += (unsigned)n;
           if (n > 0) continue;  // re-read this level: its end may have moved
         }
       }
+    }
+    if (level + 2 == k - 1 && P.leaf_batch) {
+      int taken = 0;
+      const long long tp = STATS ? clock64() : 0;
+      cnt += leaf_batch<STATS>(P, pl, S, level, it, min(end, it + kWave), L0r, L0c, lbase, balg, next, taken);
+      if (STATS) prof[2] += clock64() - tp;
+      S.it[level] = it + taken;
+      continue;
     }
     S.it[level] = it + 1;
     if (level == 0) {
@@ -549,12 +783,15 @@ __device__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S
       S.M[level + 1] = lbase[(long long)(level - 1) * P.dstride + it];
     }
     const int u = level + 2;
+    const long long tp = STATS ? clock64() : 0;
     if (u == k - 1) {
       cnt += extend<STATS>(P, pl, S, u, nullptr, lbase, true, balg, next);
+      if (STATS) prof[2] += clock64() - tp;
       continue;
     }
     int *out = lbase + (long long)level * P.dstride;  // list of level+1
     const int n = (int)extend<STATS>(P, pl, S, u, out, lbase, true, balg, next);
+    if (STATS) prof[1] += clock64() - tp;
     if (n > 0) {
       S.len[level + 1] = n;
       S.end[level + 1] = n;
@@ -579,6 +816,8 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
   unsigned long long balg = 0, next = 0, don = 0;
   int stat_pid = -1;
   bool is_idle = false;
+  long long prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const long long t_start = STATS ? clock64() : 0;
 
   for (;;) {
     // ---- acquire work: ring first, then the edge cursor, then wait ----
@@ -607,6 +846,7 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           is_idle = true;
         }
         const long long t_idle = wall_clock64();
+        const long long tpi = STATS ? clock64() : 0;
         int backoff = 1;
         for (;;) {
           int stop = 0;
@@ -633,6 +873,7 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           }
           if (lane == 0) fadd(&C->busy, -1);
         }
+        if (STATS) prof[3] += clock64() - tpi;
         if (kind == 0) break;
       }
     }
@@ -653,13 +894,15 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
     unsigned long long cnt = 0;
     if (!pl->unsat) {
       if (kind == 1) {
+        const long long tp = STATS ? clock64() : 0;
         const int n0 = filter_edges<STATS>(P, pl, e0, e1, L0r, L0c, balg);
+        if (STATS) prof[0] += clock64() - tp;
         if (n0 > 0) {
           if (STATS) next += (unsigned)n0;
           S.len[0] = n0;
           S.end[0] = n0;
           S.it[0] = 0;
-          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, cnt, balg, next, don);
+          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, cnt, balg, next, don, prof);
         }
       } else {
         const int p = t.p;
@@ -668,11 +911,13 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           if (j < p) S.M[j] = t.M[j];
         const int k = pl->k;
         unsigned long long dummy_b = 0, dummy_n = 0;
+        const long long tp = STATS ? clock64() : 0;
         for (int u = 2; u < p; u++) {
           const int n = (int)extend<STATS>(P, pl, S, u, lbase + (long long)(u - 2) * P.dstride, lbase, false,
                                            dummy_b, dummy_n);
           S.len[u - 1] = n;
         }
+        if (STATS) prof[4] += clock64() - tp;
         const bool fresh = t.fresh != 0;
         if (p == k - 1) {
           cnt += extend<STATS>(P, pl, S, p, nullptr, lbase, fresh, balg, next);
@@ -682,7 +927,7 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           S.len[p - 1] = n;
           S.it[p - 1] = t.a;
           S.end[p - 1] = min(t.b, n);
-          dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, cnt, balg, next, don);
+          dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, cnt, balg, next, don, prof);
         }
       }
     }
@@ -700,12 +945,31 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
     atomicAdd(&P.stats[stat_pid * 4 + 0], balg);
     atomicAdd(&P.stats[stat_pid * 4 + 1], next);
   }
+  if (STATS && lane == 0) {
+    prof[6] = clock64() - t_start;
+#pragma unroll
+    for (int i = 0; i < 7; i++) atomicAdd(&P.prof[i], (unsigned long long)prof[i]);
+    atomicAdd(&P.prof[7], 1ull);
+  }
+}
+
+// Sets bit y of row h for every neighbour y of hub vertex hubs[h].
+__global__ void tdfs_hub_build_kernel(const long long *rowptr, const int *colidx, const int *hubs, int nhubs,
+                                      unsigned *bits, long long words) {
+  for (int h = blockIdx.x; h < nhubs; h += gridDim.x) {
+    const int v = hubs[h];
+    unsigned *row = bits + (long long)h * words;
+    for (long long e = rowptr[v] + threadIdx.x; e < rowptr[v + 1]; e += blockDim.x) {
+      const int y = colidx[e];
+      atomicOr(row + (y >> 5), 1u << (y & 31));
+    }
+  }
 }
 
 // Resets the control block, results and ring sequence numbers (a kernel, so
 // that captured launch sequences hold no memset nodes).
 __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigned long long *stats, int nplans,
-                                 unsigned long long *seq, unsigned long long cap) {
+                                 unsigned long long *seq, unsigned long long cap, unsigned long long *prof) {
   const unsigned long long tid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned long long nth = (unsigned long long)gridDim.x * blockDim.x;
   int *cw = reinterpret_cast<int *>(ctrl);
@@ -713,6 +977,7 @@ __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigne
   for (unsigned long long i = tid; i < (unsigned long long)nplans; i += nth) counts[i] = 0;
   for (unsigned long long i = tid; i < (unsigned long long)nplans * 4; i += nth) stats[i] = 0;
   for (unsigned long long i = tid; i < cap; i += nth) seq[i] = i;
+  for (unsigned long long i = tid; i < 8; i += nth) prof[i] = 0;
 }
 
 }  // namespace
@@ -732,6 +997,10 @@ struct tdfs_graph {
   int *src;
   int *maxnbr;
   unsigned char *labels;
+  int *hub_idx;
+  unsigned *hub_bits;
+  long long hub_words;
+  int nhubs;
 };
 
 struct tdfs_ctx {
@@ -740,7 +1009,8 @@ struct tdfs_ctx {
   hipEvent_t ev0, ev1;
   int cus;
   int wallclock_khz;
-  int blocks_per_cu;
+  int blocks_per_cu;        // occupancy limit of the timed kernel
+  int blocks_per_cu_stats;  // and of the statistics kernel
   tdfs_tuning tune;
   int stats;
   // buffers grown on demand
@@ -754,6 +1024,8 @@ struct tdfs_ctx {
   unsigned long long *stats_buf;
   DevPlan *plans;
   int plan_cap;
+  unsigned long long *prof;
+  unsigned long long last_prof[8];
 };
 
 #define HIP_TRY(expr)                                                                        \
@@ -772,6 +1044,9 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->queue_log2 = 18;
   t->donate_levels = 3;
   t->reuse = 1;
+  t->leaf_batch = 1;
+  t->hub_min_degree = 256;
+  t->hub_budget_mb = 4096;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -787,6 +1062,9 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->queue_log2 > 26) t->queue_log2 = 26;
   if (t->donate_levels <= 0) t->donate_levels = d.donate_levels;
   if (t->reuse <= 0) t->reuse = d.reuse;
+  if (t->leaf_batch <= 0) t->leaf_batch = d.leaf_batch;
+  if (t->hub_min_degree <= 0) t->hub_min_degree = d.hub_min_degree;
+  if (t->hub_budget_mb == 0) t->hub_budget_mb = d.hub_budget_mb;
 }
 
 extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
@@ -809,11 +1087,13 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdfs_match_kernel<false>, kBlock, 0));
   int nb2 = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, tdfs_match_kernel<true>, kBlock, 0));
-  c->blocks_per_cu = std::max(1, std::min(nb, nb2));
+  c->blocks_per_cu = std::max(1, nb);
+  c->blocks_per_cu_stats = std::max(1, nb2);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&c->ev0));
   HIP_TRY(hipEventCreate(&c->ev1));
   HIP_TRY(hipMalloc(&c->ctrl, sizeof(Ctrl)));
+  HIP_TRY(hipMalloc(&c->prof, 8 * sizeof(unsigned long long)));
   *out = c;
   return TDFS_OK;
 }
@@ -855,6 +1135,7 @@ extern "C" void tdfs_ctx_destroy(tdfs_ctx *ctx) {
   if (ctx->q_data) (void)hipFree(ctx->q_data);
   if (ctx->q_seq) (void)hipFree(ctx->q_seq);
   if (ctx->ctrl) (void)hipFree(ctx->ctrl);
+  if (ctx->prof) (void)hipFree(ctx->prof);
   if (ctx->counts) (void)hipFree(ctx->counts);
   if (ctx->stats_buf) (void)hipFree(ctx->stats_buf);
   if (ctx->plans) (void)hipFree(ctx->plans);
@@ -921,6 +1202,44 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
       return bail("copy maxnbr", e);
     if ((e = hipMemcpy(g->labels, lab8.data(), (size_t)n, hipMemcpyHostToDevice))) return bail("copy labels", e);
   }
+  // Dense neighbour bitmaps for the highest-degree vertices, within a byte
+  // budget: membership in a hub's list becomes one load.
+  {
+    tdfs_tuning t = ctx->tune;
+    fill_defaults(&t);
+    const long long words = ((long long)n + 31) / 32;
+    std::vector<int> hubs;
+    if (t.hub_budget_mb > 0 && n > 0) {
+      for (int32_t v = 0; v < n; v++)
+        if (degv[v] >= t.hub_min_degree) hubs.push_back(v);
+      std::stable_sort(hubs.begin(), hubs.end(), [&](int a, int b) { return degv[a] > degv[b]; });
+      const long long cap = ((long long)t.hub_budget_mb << 20) / (words * 4);
+      if ((long long)hubs.size() > cap) hubs.resize((size_t)cap);
+    }
+    std::vector<int> hidx(n > 0 ? n : 1, -1);
+    for (size_t h = 0; h < hubs.size(); h++) hidx[hubs[h]] = (int)h;
+    g->nhubs = (int)hubs.size();
+    g->hub_words = words;
+    if ((e = hipMalloc(&g->hub_idx, sizeof(int) * (size_t)std::max(n, 1)))) return bail("hub_idx", e);
+    if ((e = hipMemcpy(g->hub_idx, hidx.data(), sizeof(int) * (size_t)std::max(n, 1), hipMemcpyHostToDevice)))
+      return bail("copy hub_idx", e);
+    if (!hubs.empty()) {
+      const size_t bytes = (size_t)hubs.size() * (size_t)words * 4;
+      if ((e = hipMalloc(&g->hub_bits, bytes))) return bail("hub bitmaps", e);
+      if ((e = hipMemset(g->hub_bits, 0, bytes))) return bail("clear hub bitmaps", e);
+      int *dh = nullptr;
+      if ((e = hipMalloc(&dh, sizeof(int) * hubs.size()))) return bail("hub list", e);
+      if ((e = hipMemcpy(dh, hubs.data(), sizeof(int) * hubs.size(), hipMemcpyHostToDevice))) {
+        (void)hipFree(dh);
+        return bail("copy hub list", e);
+      }
+      hipLaunchKernelGGL(tdfs_hub_build_kernel, dim3((unsigned)std::min<size_t>(hubs.size(), 65535)), dim3(256), 0,
+                         0, g->rowptr, g->colidx, dh, (int)hubs.size(), g->hub_bits, words);
+      e = hipDeviceSynchronize();
+      (void)hipFree(dh);
+      if (e) return bail("hub bitmap build", e);
+    }
+  }
   *out = g;
   return TDFS_OK;
 }
@@ -934,6 +1253,8 @@ extern "C" void tdfs_graph_free(tdfs_graph *g) {
   if (g->src) (void)hipFree(g->src);
   if (g->maxnbr) (void)hipFree(g->maxnbr);
   if (g->labels) (void)hipFree(g->labels);
+  if (g->hub_idx) (void)hipFree(g->hub_idx);
+  if (g->hub_bits) (void)hipFree(g->hub_bits);
   delete g;
 }
 
@@ -1032,7 +1353,8 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
   const unsigned long long nchunks = (unsigned long long)((span + chunk - 1) / chunk);
   const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
   const long long wstride = 2ll * kChunkMax + (long long)(kMaxK - 3) * dstride;
-  int bpc = ctx->tune.blocks_per_cu > 0 ? std::min(ctx->tune.blocks_per_cu, ctx->blocks_per_cu) : ctx->blocks_per_cu;
+  const int occ = ctx->stats ? ctx->blocks_per_cu_stats : ctx->blocks_per_cu;
+  int bpc = ctx->tune.blocks_per_cu > 0 ? std::min(ctx->tune.blocks_per_cu, occ) : occ;
   const int grid = ctx->cus * bpc;
   const long long waves = (long long)grid * kWavesPerBlock;
   int rc = ensure_buffers(ctx, g, nplans, waves, wstride);
@@ -1050,6 +1372,9 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
   P.src = g->src;
   P.maxnbr = g->maxnbr;
   P.labels = g->labels;
+  P.hub_idx = g->hub_idx;
+  P.hub_bits = g->hub_bits;
+  P.hub_words = g->hub_words;
   P.plans = ctx->plans;
   P.nplans = nplans;
   P.chunk = chunk;
@@ -1066,16 +1391,18 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
   P.counts = ctx->counts;
   P.stats = ctx->stats_buf;
   P.ctrl = ctx->ctrl;
+  P.prof = ctx->prof;
   const long long ticks_per_us = std::max(1, ctx->wallclock_khz / 1000);
   P.timeout_ticks = ctx->tune.timeout_us < 0 ? -1 : (long long)ctx->tune.timeout_us * ticks_per_us;
   P.deadline_ticks = 300ll * 1000 * 1000 * ticks_per_us;  // 5 minutes
   P.split_chunk = ctx->tune.split_chunk;
   P.donate_levels = ctx->tune.donate_levels;
   P.reuse = ctx->tune.reuse == 1 ? 1 : 0;
+  P.leaf_batch = ctx->tune.leaf_batch == 1 ? 1 : 0;
 
   const int init_blocks = (int)std::min<unsigned long long>(1024, (ctx->q_cap + 255) / 256);
   hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->ctrl, ctx->counts,
-                     ctx->stats_buf, nplans, ctx->q_seq, ctx->q_cap);
+                     ctx->stats_buf, nplans, ctx->q_seq, ctx->q_cap, ctx->prof);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   if (ctx->stats)
@@ -1091,6 +1418,7 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
   HIP_TRY(hipMemcpyAsync(stats.data(), ctx->stats_buf, sizeof(unsigned long long) * 4 * nplans,
                          hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipMemcpyAsync(&ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->last_prof, ctx->prof, sizeof(ctx->last_prof), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1111,6 +1439,12 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
       r.count = (uint64_t)prod;
     }
   }
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8) {
+  if (!ctx || !out8) return fail(TDFS_E_ARG, "null argument");
+  for (int i = 0; i < 8; i++) out8[i] = ctx->last_prof[i];
   return TDFS_OK;
 }
 

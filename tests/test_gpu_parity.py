@@ -111,6 +111,8 @@ def test_scheduling_modes_are_exact(dev):
         dict(chunk_edges=1024, timeout_us=1),
         dict(split_chunk=7, timeout_us=1, donate_levels=5),
         dict(reuse=2),
+        dict(leaf_batch=2),
+        dict(leaf_batch=2, timeout_us=1, donate_levels=5),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
@@ -136,6 +138,24 @@ def test_hub_beyond_reference_buffer(dev):
         for labeled in (0, 1):
             say("hub", text.count("v "), labeled)
             check(dev, dg, G, text, labeled)
+
+
+def test_hub_bitmaps_all_or_none(dev):
+    """Membership through neighbour bitmaps (every vertex a hub) and through
+    binary search only (no bitmaps) give identical counts and bytes."""
+    G = T.CSRGraph.chung_lu(3000, 15000, 2.1, 400, 51).with_labels(3, 2)
+    base = dev.tuning()
+    try:
+        for hub in (dict(hub_min_degree=1, hub_budget_mb=4096), dict(hub_budget_mb=-1)):
+            dev.set_tuning(**hub)
+            dg = dev.upload(G)
+            for i in (1, 2, 3, 4, 5, 6, 7, 9, 12, 14):
+                for labeled in (0, 1):
+                    say("hub bitmaps", hub, f"{i}.g", labeled)
+                    check(dev, dg, G, fixture(i), labeled)
+            dg.free()
+    finally:
+        dev.set_tuning(hub_min_degree=base.hub_min_degree, hub_budget_mb=base.hub_budget_mb)
 
 
 def test_batch_equals_alone(dev):
