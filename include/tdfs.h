@@ -86,7 +86,9 @@ typedef struct tdfs_tuning {
   int32_t leaf_batch;        /* 1: count the leaf for 64 parents at once (default); 2: one at a time */
   int32_t hub_min_degree;    /* vertices of at least this degree get a neighbour bitmap (default 256) */
   int32_t hub_budget_mb;     /* bitmap memory per uploaded graph (default 4096; <0: none); read at upload */
-  int32_t reserved[4];
+  int32_t degree_order;      /* 1: also keep a copy renumbered by ascending degree and count on it whenever
+                                the result cannot depend on the numbering (default); 2: off.  Read at upload */
+  int32_t reserved[3];
 } tdfs_tuning;
 
 /* Result of one counted pattern. */
@@ -167,6 +169,13 @@ int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4);
 int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const int64_t *rowptr,
                       const int32_t *colidx, const int32_t *labels, tdfs_graph **out);
 void tdfs_graph_free(tdfs_graph *g);
+/* out4 = {n, nnz, has degree-ordered copy, hub bitmaps of the copy counted on} */
+int tdfs_graph_info(const tdfs_graph *g, int64_t *out4);
+/* perm_out[v] = id of vertex v in the degree-ordered copy (TDFS_E_STATE if none).
+ * tdfs_count uses that copy when every plan of the call has vertex 2 adjacent
+ * to vertex 1 or dP(2) <= dP(0) (then the count is numbering-independent);
+ * edge ranges and statistics then refer to the copy's numbering. */
+int tdfs_graph_order(const tdfs_graph *g, int32_t *perm_out);
 
 /* One pattern, one launch, over directed edges [edge_begin, edge_end)
  * (edge_end < 0: all).  Blocks until done. */

@@ -95,6 +95,21 @@ class CSRGraph:
             rowptr[v + 1] = rowptr[v] + len(nb)
         return cls(rowptr, np.asarray(cols, dtype=np.int32))
 
+    def relabeled(self, perm: np.ndarray) -> "CSRGraph":
+        """The same graph with vertex v renamed perm[v] (lists re-sorted)."""
+        n = self.n
+        perm = np.asarray(perm, dtype=np.int64)
+        inv = np.empty(n, dtype=np.int64)
+        inv[perm] = np.arange(n)
+        deg = np.diff(self.rowptr)
+        rp = np.zeros(n + 1, dtype=np.int64)
+        rp[1:] = np.cumsum(deg[inv])
+        src = perm[np.repeat(np.arange(n), deg)]
+        dst = perm[self.colidx]
+        order = np.lexsort((dst, src))
+        labels = None if self.labels is None else np.ascontiguousarray(self.labels[inv]).astype(np.int32)
+        return CSRGraph(rp, dst[order].astype(np.int32), labels)
+
     def with_labels(self, nlabels: int, seed: int) -> "CSRGraph":
         labels = np.zeros(max(self.n, 1), dtype=np.int32)
         check(lib().tdfs_gen_labels(self.n, nlabels, seed, i32p(labels)), "gen labels")
@@ -136,6 +151,19 @@ class DeviceGraph:
         self.handle = C.c_void_p()
         check(lib().tdfs_graph_upload(dev.handle, g.n, g.nnz, i64p(g.rowptr), i32p(g.colidx), i32p(g.labels),
                                       C.byref(self.handle)), "graph upload")
+
+    def info(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        check(lib().tdfs_graph_info(self.handle, i64p(out)), "graph info")
+        return {"n": int(out[0]), "nnz": int(out[1]), "ordered": bool(out[2]), "hubs": int(out[3])}
+
+    def order(self) -> np.ndarray | None:
+        """New id of each vertex in the degree-ordered copy, or None."""
+        if not self.info()["ordered"]:
+            return None
+        perm = np.zeros(max(self.host.n, 1), dtype=np.int32)
+        check(lib().tdfs_graph_order(self.handle, i32p(perm)), "graph order")
+        return perm[: self.host.n]
 
     def free(self) -> None:
         if self.handle:

@@ -48,7 +48,8 @@ class Tuning(C.Structure):
         ("leaf_batch", C.c_int32),
         ("hub_min_degree", C.c_int32),
         ("hub_budget_mb", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("degree_order", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -91,6 +92,8 @@ _SIGS = {
     "tdfs_ctx_destroy": (None, [_P]),
     "tdfs_graph_upload": (C.c_int, [_P, C.c_int32, C.c_int64, _I64P, _I32P, _I32P, C.POINTER(_P)]),
     "tdfs_graph_free": (None, [_P]),
+    "tdfs_graph_info": (C.c_int, [_P, _I64P]),
+    "tdfs_graph_order": (C.c_int, [_P, _I32P]),
     "tdfs_count": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int64, C.c_int64, C.POINTER(Result)]),
     "tdfs_count_batch": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int32, C.POINTER(Result)]),
 }

@@ -985,22 +985,32 @@ __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigne
 // ---------------------------------------------------------------------------
 // C ABI (device side)
 // ---------------------------------------------------------------------------
+// One device copy of the graph under one vertex numbering.
+struct GraphView {
+  long long *rowptr = nullptr;
+  int *colidx = nullptr;
+  int *degv = nullptr;
+  int *src = nullptr;
+  int *maxnbr = nullptr;
+  unsigned char *labels = nullptr;
+  int *hub_idx = nullptr;
+  unsigned *hub_bits = nullptr;
+  long long hub_words = 0;
+  int nhubs = 0;
+};
+
 struct tdfs_graph {
   int device;
   int32_t n;
   int64_t nnz;
   int32_t dmax;
   int labeled;
-  long long *rowptr;
-  int *colidx;
-  int *degv;
-  int *src;
-  int *maxnbr;
-  unsigned char *labels;
-  int *hub_idx;
-  unsigned *hub_bits;
-  long long hub_words;
-  int nhubs;
+  // view[0]: the numbering as given.  view[1] (when `ordered`): vertices
+  // renumbered by ascending degree (ties by id), so that the symmetry-breaking
+  // "smaller id first" conditions start the DFS at low-degree vertices.
+  GraphView view[2];
+  int ordered;
+  std::vector<int32_t> perm;  // new id of each vertex in view[1]
 };
 
 struct tdfs_ctx {
@@ -1047,6 +1057,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->leaf_batch = 1;
   t->hub_min_degree = 256;
   t->hub_budget_mb = 4096;
+  t->degree_order = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1065,6 +1076,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->leaf_batch <= 0) t->leaf_batch = d.leaf_batch;
   if (t->hub_min_degree <= 0) t->hub_min_degree = d.hub_min_degree;
   if (t->hub_budget_mb == 0) t->hub_budget_mb = d.hub_budget_mb;
+  if (t->degree_order <= 0) t->degree_order = d.degree_order;
 }
 
 extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
@@ -1145,19 +1157,25 @@ extern "C" void tdfs_ctx_destroy(tdfs_ctx *ctx) {
   delete ctx;
 }
 
-extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const int64_t *rowptr,
-                                 const int32_t *colidx, const int32_t *labels, tdfs_graph **out) {
-  if (!ctx || !rowptr || (!colidx && nnz > 0) || !out || n < 0 || nnz < 0)
-    return fail(TDFS_E_ARG, "bad graph arguments");
-  int rc = tdfs_graph_validate(n, nnz, rowptr, colidx);
-  if (rc) return rc;
-  HIP_TRY(hipSetDevice(ctx->device));
+static void free_view(GraphView &v) {
+  if (v.rowptr) (void)hipFree(v.rowptr);
+  if (v.colidx) (void)hipFree(v.colidx);
+  if (v.degv) (void)hipFree(v.degv);
+  if (v.src) (void)hipFree(v.src);
+  if (v.maxnbr) (void)hipFree(v.maxnbr);
+  if (v.labels) (void)hipFree(v.labels);
+  if (v.hub_idx) (void)hipFree(v.hub_idx);
+  if (v.hub_bits) (void)hipFree(v.hub_bits);
+  v = GraphView();
+}
+
+// Copies one numbering of a validated CSR to the device with its derived
+// arrays (degree, row of each edge, max neighbour degree, uint8 labels) and
+// the dense neighbour bitmaps of its highest-degree vertices.
+static int upload_view(const tdfs_tuning &t, int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+                       const int32_t *labels, GraphView &g) {
   std::vector<int> degv(n > 0 ? n : 1), src(nnz > 0 ? nnz : 1), maxnbr(n > 0 ? n : 1);
-  int32_t dmax = 0;
-  for (int32_t v = 0; v < n; v++) {
-    degv[v] = (int)(rowptr[v + 1] - rowptr[v]);
-    dmax = std::max(dmax, degv[v]);
-  }
+  for (int32_t v = 0; v < n; v++) degv[v] = (int)(rowptr[v + 1] - rowptr[v]);
   for (int32_t v = 0; v < n; v++) {
     int m = -1;
     for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) {
@@ -1166,79 +1184,121 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
     }
     maxnbr[v] = m;
   }
-  tdfs_graph *g = new tdfs_graph();
-  std::memset(g, 0, sizeof(*g));
-  g->device = ctx->device;
-  g->n = n;
-  g->nnz = nnz;
-  g->dmax = dmax;
-  g->labeled = labels != nullptr;
   auto bail = [&](const char *what, hipError_t e) {
-    tdfs_graph_free(g);
+    free_view(g);
     return fail(TDFS_E_OOM, "%s: %s", what, hipGetErrorString(e));
   };
   hipError_t e;
-  if ((e = hipMalloc(&g->rowptr, sizeof(long long) * (size_t)(n + 1)))) return bail("rowptr", e);
-  if ((e = hipMalloc(&g->colidx, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("colidx", e);
-  if ((e = hipMalloc(&g->degv, sizeof(int) * (size_t)std::max(n, 1)))) return bail("degv", e);
-  if ((e = hipMalloc(&g->src, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("src", e);
-  if ((e = hipMalloc(&g->maxnbr, sizeof(int) * (size_t)std::max(n, 1)))) return bail("maxnbr", e);
-  if ((e = hipMalloc(&g->labels, (size_t)std::max(n, 1)))) return bail("labels", e);
+  if ((e = hipMalloc(&g.rowptr, sizeof(long long) * (size_t)(n + 1)))) return bail("rowptr", e);
+  if ((e = hipMalloc(&g.colidx, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("colidx", e);
+  if ((e = hipMalloc(&g.degv, sizeof(int) * (size_t)std::max(n, 1)))) return bail("degv", e);
+  if ((e = hipMalloc(&g.src, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("src", e);
+  if ((e = hipMalloc(&g.maxnbr, sizeof(int) * (size_t)std::max(n, 1)))) return bail("maxnbr", e);
+  if ((e = hipMalloc(&g.labels, (size_t)std::max(n, 1)))) return bail("labels", e);
   std::vector<unsigned char> lab8(n > 0 ? n : 1, 255);
   if (labels)
     for (int32_t v = 0; v < n; v++) lab8[v] = (labels[v] >= 0 && labels[v] < 255) ? (unsigned char)labels[v] : 255;
-  if ((e = hipMemcpy(g->rowptr, rowptr, sizeof(long long) * (size_t)(n + 1), hipMemcpyHostToDevice)))
+  if ((e = hipMemcpy(g.rowptr, rowptr, sizeof(long long) * (size_t)(n + 1), hipMemcpyHostToDevice)))
     return bail("copy rowptr", e);
   if (nnz > 0) {
-    if ((e = hipMemcpy(g->colidx, colidx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice)))
+    if ((e = hipMemcpy(g.colidx, colidx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice)))
       return bail("copy colidx", e);
-    if ((e = hipMemcpy(g->src, src.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice)))
-      return bail("copy src", e);
+    if ((e = hipMemcpy(g.src, src.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice))) return bail("copy src", e);
   }
   if (n > 0) {
-    if ((e = hipMemcpy(g->degv, degv.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
-      return bail("copy degv", e);
-    if ((e = hipMemcpy(g->maxnbr, maxnbr.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
+    if ((e = hipMemcpy(g.degv, degv.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice))) return bail("copy degv", e);
+    if ((e = hipMemcpy(g.maxnbr, maxnbr.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
       return bail("copy maxnbr", e);
-    if ((e = hipMemcpy(g->labels, lab8.data(), (size_t)n, hipMemcpyHostToDevice))) return bail("copy labels", e);
+    if ((e = hipMemcpy(g.labels, lab8.data(), (size_t)n, hipMemcpyHostToDevice))) return bail("copy labels", e);
   }
   // Dense neighbour bitmaps for the highest-degree vertices, within a byte
   // budget: membership in a hub's list becomes one load.
-  {
-    tdfs_tuning t = ctx->tune;
-    fill_defaults(&t);
-    const long long words = ((long long)n + 31) / 32;
-    std::vector<int> hubs;
-    if (t.hub_budget_mb > 0 && n > 0) {
-      for (int32_t v = 0; v < n; v++)
-        if (degv[v] >= t.hub_min_degree) hubs.push_back(v);
-      std::stable_sort(hubs.begin(), hubs.end(), [&](int a, int b) { return degv[a] > degv[b]; });
-      const long long cap = ((long long)t.hub_budget_mb << 20) / (words * 4);
-      if ((long long)hubs.size() > cap) hubs.resize((size_t)cap);
-    }
-    std::vector<int> hidx(n > 0 ? n : 1, -1);
-    for (size_t h = 0; h < hubs.size(); h++) hidx[hubs[h]] = (int)h;
-    g->nhubs = (int)hubs.size();
-    g->hub_words = words;
-    if ((e = hipMalloc(&g->hub_idx, sizeof(int) * (size_t)std::max(n, 1)))) return bail("hub_idx", e);
-    if ((e = hipMemcpy(g->hub_idx, hidx.data(), sizeof(int) * (size_t)std::max(n, 1), hipMemcpyHostToDevice)))
-      return bail("copy hub_idx", e);
-    if (!hubs.empty()) {
-      const size_t bytes = (size_t)hubs.size() * (size_t)words * 4;
-      if ((e = hipMalloc(&g->hub_bits, bytes))) return bail("hub bitmaps", e);
-      if ((e = hipMemset(g->hub_bits, 0, bytes))) return bail("clear hub bitmaps", e);
-      int *dh = nullptr;
-      if ((e = hipMalloc(&dh, sizeof(int) * hubs.size()))) return bail("hub list", e);
-      if ((e = hipMemcpy(dh, hubs.data(), sizeof(int) * hubs.size(), hipMemcpyHostToDevice))) {
-        (void)hipFree(dh);
-        return bail("copy hub list", e);
-      }
-      hipLaunchKernelGGL(tdfs_hub_build_kernel, dim3((unsigned)std::min<size_t>(hubs.size(), 65535)), dim3(256), 0,
-                         0, g->rowptr, g->colidx, dh, (int)hubs.size(), g->hub_bits, words);
-      e = hipDeviceSynchronize();
+  const long long words = ((long long)n + 31) / 32;
+  std::vector<int> hubs;
+  if (t.hub_budget_mb > 0 && n > 0) {
+    for (int32_t v = 0; v < n; v++)
+      if (degv[v] >= t.hub_min_degree) hubs.push_back(v);
+    std::stable_sort(hubs.begin(), hubs.end(), [&](int a, int b) { return degv[a] > degv[b]; });
+    const long long cap = ((long long)t.hub_budget_mb << 20) / (words * 4);
+    if ((long long)hubs.size() > cap) hubs.resize((size_t)cap);
+  }
+  std::vector<int> hidx(n > 0 ? n : 1, -1);
+  for (size_t h = 0; h < hubs.size(); h++) hidx[hubs[h]] = (int)h;
+  g.nhubs = (int)hubs.size();
+  g.hub_words = words;
+  if ((e = hipMalloc(&g.hub_idx, sizeof(int) * (size_t)std::max(n, 1)))) return bail("hub_idx", e);
+  if ((e = hipMemcpy(g.hub_idx, hidx.data(), sizeof(int) * (size_t)std::max(n, 1), hipMemcpyHostToDevice)))
+    return bail("copy hub_idx", e);
+  if (!hubs.empty()) {
+    const size_t bytes = (size_t)hubs.size() * (size_t)words * 4;
+    if ((e = hipMalloc(&g.hub_bits, bytes))) return bail("hub bitmaps", e);
+    if ((e = hipMemset(g.hub_bits, 0, bytes))) return bail("clear hub bitmaps", e);
+    int *dh = nullptr;
+    if ((e = hipMalloc(&dh, sizeof(int) * hubs.size()))) return bail("hub list", e);
+    if ((e = hipMemcpy(dh, hubs.data(), sizeof(int) * hubs.size(), hipMemcpyHostToDevice))) {
       (void)hipFree(dh);
-      if (e) return bail("hub bitmap build", e);
+      return bail("copy hub list", e);
     }
+    hipLaunchKernelGGL(tdfs_hub_build_kernel, dim3((unsigned)std::min<size_t>(hubs.size(), 65535)), dim3(256), 0, 0,
+                       g.rowptr, g.colidx, dh, (int)hubs.size(), g.hub_bits, words);
+    e = hipDeviceSynchronize();
+    (void)hipFree(dh);
+    if (e) return bail("hub bitmap build", e);
+  }
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const int64_t *rowptr,
+                                 const int32_t *colidx, const int32_t *labels, tdfs_graph **out) {
+  if (!ctx || !rowptr || (!colidx && nnz > 0) || !out || n < 0 || nnz < 0)
+    return fail(TDFS_E_ARG, "bad graph arguments");
+  int rc = tdfs_graph_validate(n, nnz, rowptr, colidx);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  tdfs_tuning t = ctx->tune;
+  fill_defaults(&t);
+  tdfs_graph *g = new tdfs_graph();
+  g->device = ctx->device;
+  g->n = n;
+  g->nnz = nnz;
+  g->dmax = 0;
+  for (int32_t v = 0; v < n; v++) g->dmax = std::max<int32_t>(g->dmax, (int32_t)(rowptr[v + 1] - rowptr[v]));
+  g->labeled = labels != nullptr;
+  g->ordered = 0;
+  rc = upload_view(t, n, nnz, rowptr, colidx, labels, g->view[0]);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  if (t.degree_order == 1 && n > 0) {
+    // new id = rank by (degree, id)
+    std::vector<int32_t> byrank(n);
+    for (int32_t v = 0; v < n; v++) byrank[v] = v;
+    std::stable_sort(byrank.begin(), byrank.end(), [&](int32_t a, int32_t b) {
+      return rowptr[a + 1] - rowptr[a] < rowptr[b + 1] - rowptr[b];
+    });
+    g->perm.assign(n, 0);
+    for (int32_t r = 0; r < n; r++) g->perm[byrank[r]] = r;
+    std::vector<int64_t> rp2(n + 1, 0);
+    for (int32_t r = 0; r < n; r++) rp2[r + 1] = rp2[r] + (rowptr[byrank[r] + 1] - rowptr[byrank[r]]);
+    std::vector<int32_t> ci2(nnz > 0 ? nnz : 1), lab2;
+    for (int32_t r = 0; r < n; r++) {
+      const int32_t v = byrank[r];
+      int32_t *dst = ci2.data() + rp2[r];
+      for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) *dst++ = g->perm[colidx[e]];
+      std::sort(ci2.data() + rp2[r], ci2.data() + rp2[r + 1]);
+    }
+    if (labels) {
+      lab2.resize(n);
+      for (int32_t v = 0; v < n; v++) lab2[g->perm[v]] = labels[v];
+    }
+    rc = upload_view(t, n, nnz, rp2.data(), ci2.data(), labels ? lab2.data() : nullptr, g->view[1]);
+    if (rc) {
+      free_view(g->view[0]);
+      delete g;
+      return rc;
+    }
+    g->ordered = 1;
   }
   *out = g;
   return TDFS_OK;
@@ -1247,16 +1307,32 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
 extern "C" void tdfs_graph_free(tdfs_graph *g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
-  if (g->rowptr) (void)hipFree(g->rowptr);
-  if (g->colidx) (void)hipFree(g->colidx);
-  if (g->degv) (void)hipFree(g->degv);
-  if (g->src) (void)hipFree(g->src);
-  if (g->maxnbr) (void)hipFree(g->maxnbr);
-  if (g->labels) (void)hipFree(g->labels);
-  if (g->hub_idx) (void)hipFree(g->hub_idx);
-  if (g->hub_bits) (void)hipFree(g->hub_bits);
+  free_view(g->view[0]);
+  free_view(g->view[1]);
   delete g;
 }
+
+extern "C" int tdfs_graph_info(const tdfs_graph *g, int64_t *out4) {
+  if (!g || !out4) return fail(TDFS_E_ARG, "null argument");
+  out4[0] = g->n;
+  out4[1] = g->nnz;
+  out4[2] = g->ordered;
+  out4[3] = g->ordered ? g->view[1].nhubs : g->view[0].nhubs;
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_graph_order(const tdfs_graph *g, int32_t *perm_out) {
+  if (!g || !perm_out) return fail(TDFS_E_ARG, "null argument");
+  if (!g->ordered) return fail(TDFS_E_STATE, "graph was uploaded without the degree-ordered view");
+  std::memcpy(perm_out, g->perm.data(), sizeof(int32_t) * (size_t)g->n);
+  return TDFS_OK;
+}
+
+// The level-0 prune "some neighbour of M(1) has degree >= dP(2)"
+// (src/gpu_match.cu:540-548) never drops an embedding when vertex 2 is
+// adjacent to vertex 1 or dP(2) <= dP(0); only then is the raw count
+// independent of the vertex numbering.
+static bool prune_is_sound(const tdfs_plan &p) { return p.adj[2][1] != 0 || p.deg[2] <= p.deg[0]; }
 
 static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
   std::memset(&d, 0, sizeof(d));
@@ -1366,15 +1442,18 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
 
   KParams P;
   std::memset(&P, 0, sizeof(P));
-  P.rowptr = g->rowptr;
-  P.colidx = g->colidx;
-  P.degv = g->degv;
-  P.src = g->src;
-  P.maxnbr = g->maxnbr;
-  P.labels = g->labels;
-  P.hub_idx = g->hub_idx;
-  P.hub_bits = g->hub_bits;
-  P.hub_words = g->hub_words;
+  bool all_sound = true;
+  for (int i = 0; i < nplans; i++) all_sound = all_sound && prune_is_sound(plans[i]);
+  const GraphView &V = (g->ordered && all_sound) ? g->view[1] : g->view[0];
+  P.rowptr = V.rowptr;
+  P.colidx = V.colidx;
+  P.degv = V.degv;
+  P.src = V.src;
+  P.maxnbr = V.maxnbr;
+  P.labels = V.labels;
+  P.hub_idx = V.hub_idx;
+  P.hub_bits = V.hub_bits;
+  P.hub_words = V.hub_words;
   P.plans = ctx->plans;
   P.nplans = nplans;
   P.chunk = chunk;

@@ -250,3 +250,25 @@ def test_unsound_prune_order_is_reference_semantics():
     got = O.count(G.rowptr, G.colidx, None, p)["count"]
     assert got == restricted
     assert restricted <= full
+
+
+def test_counts_invariant_under_renumbering():
+    """ulb = #monomorphisms and lb = #label-preserving monomorphisms do not
+    depend on the vertex numbering of the data graph (for orders whose
+    level-0 prune is sound), while the DFS tree and its bytes do."""
+    G = T.CSRGraph.chung_lu(4000, 20000, 2.1, 300, 12).with_labels(3, 4)
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(G.n)
+    H = G.relabeled(perm)
+    H.validate()
+    deg_order = np.empty(G.n, dtype=np.int64)
+    deg_order[np.argsort(G.degrees(), kind="stable")] = np.arange(G.n)
+    D = G.relabeled(deg_order)
+    for text in [fixture(i) for i in range(1, 15)] + _random_patterns(20, 21, kmax=6):
+        for labeled in (0, 1):
+            p = O.plan(text, labeled)
+            if not (p.adj[2][1] or p.deg[2] <= p.deg[0]):
+                continue
+            counts = [O.count(X.rowptr, X.colidx, X.labels if labeled else None, p, threads=4)["raw"]
+                      for X in (G, H, D)]
+            assert counts[0] == counts[1] == counts[2], (text, labeled)
