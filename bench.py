@@ -1,20 +1,20 @@
-"""Benchmark: T-DFS subgraph counting on one MI355X.
+"""Benchmark: T-DFS subgraph counting on one MI355X, batched vs one pattern per launch.
 
-Workload (default "c2"): unlabeled Chung-Lu graph, n = 1.1M, m = 3.0M
-undirected edges, degree exponent 2.1, largest expected degree 30k, seed 2,
-resident in HBM; patterns 4-clique and diamond (the reference's
-data/pattern/2.g and 1.g), exact counts.
+Headline workload (default "c4"): the 64 seeded patterns of
+tests/golden/c4_patterns.json (3-6 vertices, half labeled with 4 labels; rule
+in tdfs_amd/workloads.py) on the labeled Chung-Lu graph n = 1.1M, m = 3.0M,
+gamma = 2.1, seed 2 (labels seed 3), resident in HBM, exact counts.
 
 One step = every pattern counted once over the whole graph.
-  batched : all patterns in ONE persistent launch (tdfs_count_batch)
+  batched : all patterns in ONE persistent launch, replayed from a hipGraph
   plain   : one launch per pattern (tdfs_count), the reference's form
-value = total matches / batched step time  (matches/s, higher is better).
+value = total matches / batched step time (matches/s, higher is better).
 
-Also reported: the plain form's matches/s, the roofline of the match kernel
-(algorithmic bytes counted by the kernel itself / hipEvent kernel time), and
-the CPU oracle on a bounded sample of the same workload.
+Also reported: the plain form, the c2 workload (4-clique + diamond, unlabeled),
+the roofline of the match kernel (algorithmic bytes counted by the kernel
+itself / hipEvent kernel time), and the CPU oracle on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload c4|c2|c3|c1] [--no-cpu] [--no-c2]
 """
 from __future__ import annotations
 
@@ -32,66 +32,117 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
-TRIANGLE = "v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n"
 
+def load_workload(name: str):
+    """(graph, {name: (.g text, labeled)}, description)"""
+    from tdfs_amd import workloads as W
 
-def workload(name: str):
-    import tdfs_amd as T
-
-    pat = ROOT / "tests" / "golden" / "patterns"
-    if name == "c2":
-        g = T.CSRGraph.chung_lu(1_100_000, 3_000_000, 2.1, 30000.0, 2)
-        pats = {"4-clique": (pat / "2.g").read_text(), "diamond": (pat / "1.g").read_text()}
-        desc = "Chung-Lu n=1.1M m=3.0M gamma=2.1 dmax~30k seed 2, unlabeled; 4-clique + diamond"
-    elif name == "c1":
-        g = T.CSRGraph.erdos_renyi(20_000, 100_000, 1)
-        pats = {"triangle": TRIANGLE}
-        desc = "Erdos-Renyi n=20000 m=100000 seed 1, unlabeled; triangle"
+    g = W.graph(name)
+    if name == "c4":
+        d = json.loads((ROOT / "tests" / "golden" / "c4_patterns.json").read_text())
+        pats = {f"p{p['index']:02d}_k{p['k']}{'L' if p['labeled'] else ''}": (p["text"], p["labeled"])
+                for p in d["patterns"]}
+        desc = ("64 seeded patterns (k 3-6, 32 labeled) on Chung-Lu n=1.1M m=3.0M gamma=2.1 seed 2, "
+                "4 labels seed 3")
     else:
-        raise SystemExit(f"unknown workload {name}")
-    plans = {k: T.plan_pattern(v, 0) for k, v in pats.items()}
-    return g, pats, plans, desc
+        pats = W.patterns(name)
+        desc = {"c1": "Erdos-Renyi n=20000 m=100000 seed 1; triangle",
+                "c2": "Chung-Lu n=1.1M m=3.0M gamma=2.1 dmax~30k seed 2, unlabeled; 4-clique + diamond",
+                "c3": "c2 graph + 4 labels seed 3; labeled house + 5-clique"}[name]
+    return g, pats, desc
 
 
 def cpu_baseline(g, pats, threads: int, budget_s: float = 15.0) -> dict:
-    """The oracle on seeded random windows of level-0 edges: the sample is
-    doubled until one pass over it takes about budget_s / 4 of wall time;
-    matches/s = sampled matches / time of the final pass."""
+    """The oracle on seeded random windows of level-0 edges, 16 windows at a
+    time, until budget_s of wall time has been spent; matches/s = sampled
+    matches / time spent."""
     from oracle import oracle as O
 
     rng = np.random.default_rng(1234)
     win = 256
-    oplans = {k: O.plan(v, 0) for k, v in pats.items()}
-    nwin = 16
-    while True:
-        starts = np.sort(rng.choice(max(1, g.nnz // win), size=min(nwin, g.nnz // win), replace=False)) * win
+    oplans = [O.plan(t, lab) for t, lab in pats.values()]
+    prep = O.Prepared(g.rowptr, g.colidx, g.labels)
+    order = rng.permutation(max(1, g.nnz // win))
+    matches, spent, used = 0, 0.0, 0
+    while spent < budget_s and used < order.shape[0]:
+        starts = np.sort(order[used:used + 16]) * win
+        used += starts.shape[0]
         t0 = time.perf_counter()
-        matches = 0
-        for p in oplans.values():
-            matches += O.count_ranges(g.rowptr, g.colidx, None, p, starts, starts + win, threads=threads)["count"]
-        dt = time.perf_counter() - t0
-        if dt > budget_s / 4 or nwin >= g.nnz // win:
-            break
-        nwin *= 2
-    edges = int(starts.shape[0]) * win
+        for p in oplans:
+            matches += prep.count_ranges(p, starts, starts + win, threads=threads)["count"]
+        spent += time.perf_counter() - t0
+    prep.close()
+    edges = used * win
     return {
-        "value": matches / dt,
+        "value": matches / spent,
         "unit": "matches/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{starts.shape[0]} seeded random windows of {win} level-0 edges "
-                  f"({100.0 * edges / g.nnz:.2f}% of the directed edges), every pattern, "
-                  f"oracle/tdfs_oracle.c on {threads} threads, {dt:.1f} s",
+        "sample": f"{used} seeded random windows of {win} level-0 edges "
+                  f"({100.0 * edges / g.nnz:.2f}% of the directed edges), all {len(oplans)} patterns, "
+                  f"oracle/tdfs_oracle.c on {threads} threads, {spent:.1f} s",
+    }
+
+
+def measure(dev, dg, pats, steps: int, warmup: int) -> dict:
+    """Exact counts + algorithmic bytes (statistics launch), then timed
+    batched (hipGraph replay) and plain (one launch per pattern) steps."""
+    import tdfs_amd as T
+
+    names = list(pats)
+    plans = [T.plan_pattern(t, lab) for t, lab in pats.values()]
+    dev.set_stats(True)
+    st = dev.count_batch(dg, plans)
+    prof = dev.last_profile()
+    dev.set_stats(False)
+    counts = [int(r.count) for r in st]
+    alg_bytes = sum(int(r.alg_bytes) for r in st)
+    batch = T.Batch(dev, dg, plans)
+
+    def batched():
+        res = batch.run()
+        got = [int(r.count) for r in res]
+        assert got == counts, "batched counts differ from the statistics launch"
+        return res[0].kernel_ms
+
+    def plain():
+        ms = 0.0
+        for p, c in zip(plans, counts):
+            r = dev.count(dg, p)
+            assert int(r.count) == c
+            ms += r.kernel_ms
+        return ms
+
+    for _ in range(warmup):
+        batched()
+        plain()
+    kb, kp = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kb.append(batched())
+    t_b = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kp.append(plain())
+    t_p = (time.perf_counter() - t0) / steps
+    batch.close()
+    total = sum(counts)
+    kernel_ms = float(np.mean(kb))
+    return {
+        "names": names, "counts": dict(zip(names, counts)), "total": total, "alg_bytes": alg_bytes,
+        "t_batched": t_b, "t_plain": t_p, "kernel_ms_batched": kernel_ms, "kernel_ms_plain": float(np.mean(kp)),
+        "achieved_gbs": alg_bytes / (kernel_ms * 1e-3) / 1e9, "profile": prof,
     }
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c4")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -102,86 +153,71 @@ def main() -> int:
 
     import tdfs_amd as T
 
-    g, pats, plans, desc = workload(args.workload)
+    tuned = {}
+    tfile = ROOT / "profiles" / "autotune.json"
+    if tfile.exists():
+        tuned = json.loads(tfile.read_text()).get(args.workload, {}).get("best", {})
+
+    g, pats, desc = load_workload(args.workload)
     dev = T.Device(0)
+    if tuned:
+        dev.set_tuning(**tuned)
     dg = dev.upload(g)
-    names = list(plans)
-    plist = [plans[k] for k in names]
-
-    # statistics launch: exact counts + algorithmic bytes per pattern
-    dev.set_stats(True)
-    stats = {k: dev.count(dg, plans[k]) for k in names}
-    dev.set_stats(False)
-    counts = {k: int(stats[k].count) for k in names}
-    alg_bytes = sum(int(stats[k].alg_bytes) for k in names)
-    total_matches = sum(counts.values())
-
-    def batched():
-        res = dev.count_batch(dg, plist)
-        for k, r in zip(names, res):
-            assert int(r.count) == counts[k], (k, r.count, counts[k])
-        return res[0].kernel_ms
-
-    def plain():
-        ms = 0.0
-        for k in names:
-            r = dev.count(dg, plans[k])
-            assert int(r.count) == counts[k]
-            ms += r.kernel_ms
-        return ms
-
-    for _ in range(args.warmup):
-        batched()
-        plain()
-    kb, kp = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kb.append(batched())
-    t_batched = (time.perf_counter() - t0) / args.steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kp.append(plain())
-    t_plain = (time.perf_counter() - t0) / args.steps
-    kernel_ms = float(np.mean(kb))
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    m = measure(dev, dg, pats, args.steps, args.warmup)
+    dg.free()
 
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    trfile = ROOT / "profiles" / "traffic.json"
+    if trfile.exists():
+        traffic = json.loads(trfile.read_text()).get(args.workload, {}).get("hbm_bytes_per_launch")
+
+    extra = {}
+    if args.workload != "c2" and not args.no_c2:
+        g2, p2, d2 = load_workload("c2")
+        dg2 = dev.upload(g2)
+        m2 = measure(dev, dg2, p2, args.steps, args.warmup)
+        dg2.free()
+        extra = {"c2": {"workload": d2, "counts": m2["counts"], "batched_matches_per_s": m2["total"] / m2["t_batched"],
+                        "plain_matches_per_s": m2["total"] / m2["t_plain"], "ms_per_step": m2["t_batched"] * 1e3,
+                        "kernel_ms_batched": m2["kernel_ms_batched"],
+                        "achieved_alg_gbs": m2["achieved_gbs"]}}
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(g, pats, threads)
 
+    prof = m["profile"]
+    tot = max(1, prof["total"])
     line = {
-        "metric": "matches/sec (batched launch)",
-        "value": total_matches / t_batched,
+        "metric": "matches/sec (batched launch vs one pattern per launch)",
+        "value": m["total"] / m["t_batched"],
         "unit": "matches/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_batched * 1e3,
+        "ms_per_step": m["t_batched"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded in-repo generator)",
-        "config": {"workload": f"{args.workload}: {desc}", "patterns": names, "counts": counts,
-                   "graph": {"n": g.n, "nnz": g.nnz, "dmax": g.dmax}},
-        "plain_matches_per_s": total_matches / t_plain,
-        "plain_ms_per_step": t_plain * 1e3,
-        "batched_speedup": t_plain / t_batched,
-        "kernel_ms_batched": kernel_ms,
-        "kernel_ms_plain": float(np.mean(kp)),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes, "kernel": "tdfs_match_kernel"},
+        "config": {"workload": f"{args.workload}: {desc}", "patterns": len(m["names"]),
+                   "total_matches": m["total"], "graph": {"n": g.n, "nnz": g.nnz, "dmax": g.dmax},
+                   "tuning": tuned or "defaults"},
+        "plain_matches_per_s": m["total"] / m["t_plain"],
+        "plain_ms_per_step": m["t_plain"] * 1e3,
+        "batched_speedup": m["t_plain"] / m["t_batched"],
+        "kernel_ms_batched": m["kernel_ms_batched"],
+        "kernel_ms_plain_sum": m["kernel_ms_plain"],
+        "roofline": {"bound": "hbm", "achieved": m["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": m["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": m["alg_bytes"], "kernel": "tdfs_match_kernel"},
+        "phase_cycles_frac": {k: round(v / tot, 4) for k, v in prof.items() if k not in ("total", "waves")},
         "cpu_baseline": cpu,
+        **extra,
     }
     print(json.dumps(line))
-    dg.free()
     dev.close()
     return 0
 

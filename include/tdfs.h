@@ -148,6 +148,16 @@ int tdfs_gen_labels(int32_t n, int32_t nlabels, uint64_t seed, int32_t *labels);
 int tdfs_gen_pattern(int32_t k, int32_t m, int32_t nlabels, uint64_t seed, char *text_out,
                      size_t cap);
 
+/* ---------------- host side: work estimate ---------------- */
+/* Expands the counting DFS from nsample uniformly drawn directed edges
+ * (seeded) and scales by nnz / nsample.  out3 = {estimated cost in list
+ * elements tested, estimated raw count, 1 if the scaled cost passed cap and
+ * the expansion stopped (then the cost is a lower bound)}.  Used for seeded
+ * feasibility filters and serving admission. */
+int tdfs_estimate_work(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+                       const int32_t *labels, const tdfs_plan *plan, int32_t nsample, uint64_t seed,
+                       double cap, double *out3);
+
 /* ---------------- device side ---------------- */
 void tdfs_tuning_default(tdfs_tuning *t);
 int tdfs_ctx_create(int32_t device, tdfs_ctx **out);
@@ -176,6 +186,9 @@ int tdfs_graph_info(const tdfs_graph *g, int64_t *out4);
  * to vertex 1 or dP(2) <= dP(0) (then the count is numbering-independent);
  * edge ranges and statistics then refer to the copy's numbering. */
 int tdfs_graph_order(const tdfs_graph *g, int32_t *perm_out);
+/* tdfs_estimate_work on the numbering tdfs_count would use for this plan. */
+int tdfs_graph_estimate(const tdfs_graph *g, const tdfs_plan *plan, int32_t nsample, uint64_t seed,
+                        double cap, double *out3);
 
 /* One pattern, one launch, over directed edges [edge_begin, edge_end)
  * (edge_end < 0: all).  Blocks until done. */
@@ -185,6 +198,21 @@ int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_
  * out[i] per plan, kernel_ms is the launch's total in every entry. */
 int tdfs_count_batch(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
                      tdfs_result *out);
+
+/* ---------------- prepared batches (hipGraph) ---------------- */
+typedef struct tdfs_batch tdfs_batch;
+/* Uploads the plans once and captures the whole launch sequence (reset
+ * kernel, persistent match kernel, result copies) into a hipGraph on a stream
+ * of the batch's own; every buffer belongs to the batch, so batches replay
+ * concurrently with each other and with tdfs_count on the same device.
+ * blocks_per_cu > 0 caps the persistent grid (to share the device). */
+int tdfs_batch_create(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
+                      int32_t blocks_per_cu, tdfs_batch **out);
+int tdfs_batch_launch(tdfs_batch *b);                 /* start one replay, do not wait */
+int tdfs_batch_done(const tdfs_batch *b);             /* 1 finished, 0 running, <0 error */
+int tdfs_batch_wait(tdfs_batch *b, tdfs_result *out); /* wait, then results per plan */
+int tdfs_batch_run(tdfs_batch *b, tdfs_result *out);  /* launch + wait */
+void tdfs_batch_destroy(tdfs_batch *b);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,12 @@ def lib():
         L.orc_count_ranges.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(OrcPlan), C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int64), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_prepare.restype = C.c_void_p
+        L.orc_prepare.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]
+        L.orc_release.argtypes = [C.c_void_p]
+        L.orc_count_prepared.argtypes = [C.c_void_p, C.POINTER(OrcPlan), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int64), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
         L.orc_bruteforce_count.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                            C.POINTER(C.c_int32), C.POINTER(OrcPlan)]
         L.orc_bruteforce_count.restype = C.c_uint64
@@ -121,6 +127,40 @@ def count_ranges(rowptr, colidx, labels, p: OrcPlan, los, his, threads: int = 1,
     raw = int(out[0])
     return {"raw": raw, "count": raw if p.labeled else raw * int(p.mult), "alg_bytes": int(out[1]),
             "extensions": int(sum(out[2:9]))}
+
+
+class Prepared:
+    """A graph prepared once for many oracle counts (keeps its arrays alive)."""
+
+    def __init__(self, rowptr, colidx, labels=None):
+        self.rp, self.ci, self.lb = _ptrs(rowptr, colidx, labels)
+        self.h = lib().orc_prepare(self.rp.shape[0] - 1, self.ci.shape[0],
+                                   self.rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   self.ci.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   None if self.lb is None else self.lb.ctypes.data_as(C.POINTER(C.c_int32)))
+
+    def count_ranges(self, p: OrcPlan, los, his, threads: int = 1, reuse: bool = True) -> dict:
+        lo = np.ascontiguousarray(los, dtype=np.int64)
+        hi = np.ascontiguousarray(his, dtype=np.int64)
+        out = (C.c_uint64 * 9)()
+        rc = lib().orc_count_prepared(self.h, C.byref(p), lo.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      hi.ctypes.data_as(C.POINTER(C.c_int64)), lo.shape[0], threads, int(reuse), out)
+        if rc:
+            raise ValueError(f"oracle count failed ({rc})")
+        raw = int(out[0])
+        return {"raw": raw, "count": raw if p.labeled else raw * int(p.mult), "alg_bytes": int(out[1]),
+                "extensions": int(sum(out[2:9]))}
+
+    def close(self):
+        if self.h:
+            lib().orc_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bruteforce(rowptr, colidx, labels, p: OrcPlan) -> int:

@@ -474,13 +474,51 @@ static void *thread_main(void *arg) {
  * out[1] = algorithmic bytes
  * out[2+u], u = 0..6: extensions computing C(u) (u = 1: accepted level-0 edges)
  * Returns 0, or a negative code. */
+/* A graph prepared once (row ids, max neighbour degree) for many counts. */
+void *orc_prepare(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+                  const int32_t *labels) {
+  orc_graph *g = (orc_graph *)calloc(1, sizeof(orc_graph));
+  g->n = n;
+  g->nnz = nnz;
+  g->rowptr = rowptr;
+  g->colidx = colidx;
+  g->labels = labels;
+  graph_prepare(g);
+  return g;
+}
+
+void orc_release(void *h) {
+  if (!h) return;
+  graph_release((orc_graph *)h);
+  free(h);
+}
+
+static int count_prepared(const orc_graph *gp, const orc_plan *p, const int64_t *lo_in,
+                          const int64_t *hi_in, int nranges, int nthreads, int reuse, uint64_t *out);
+
 int orc_count_ranges(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
                      const int32_t *labels, const orc_plan *p, const int64_t *lo_in,
                      const int64_t *hi_in, int nranges, int nthreads, int reuse, uint64_t *out) {
   if (p->labeled && !labels) return -1;
   if (nranges < 1) return -2;
-  orc_graph g = {n, nnz, rowptr, colidx, labels, NULL, NULL, 0};
-  graph_prepare(&g);
+  orc_graph *g = (orc_graph *)orc_prepare(n, nnz, rowptr, colidx, labels);
+  int rc = count_prepared(g, p, lo_in, hi_in, nranges, nthreads, reuse, out);
+  orc_release(g);
+  return rc;
+}
+
+int orc_count_prepared(const void *h, const orc_plan *p, const int64_t *lo_in, const int64_t *hi_in,
+                       int nranges, int nthreads, int reuse, uint64_t *out) {
+  const orc_graph *g = (const orc_graph *)h;
+  if (p->labeled && !g->labels) return -1;
+  if (nranges < 1) return -2;
+  return count_prepared(g, p, lo_in, hi_in, nranges, nthreads, reuse, out);
+}
+
+static int count_prepared(const orc_graph *gp, const orc_plan *p, const int64_t *lo_in,
+                          const int64_t *hi_in, int nranges, int nthreads, int reuse, uint64_t *out) {
+  const orc_graph g = *gp;
+  const int64_t nnz = g.nnz;
   int64_t *lo = (int64_t *)malloc(sizeof(int64_t) * (size_t)nranges);
   int64_t *hi = (int64_t *)malloc(sizeof(int64_t) * (size_t)nranges);
   int64_t *pre = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nranges + 1));
@@ -528,7 +566,6 @@ int orc_count_ranges(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_
   free(lo);
   free(hi);
   free(pre);
-  graph_release(&g);
   return 0;
 }
 

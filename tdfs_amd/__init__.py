@@ -16,8 +16,8 @@ import numpy as np
 from ._lib import Plan, Result, Tuning, check, i32p, i64p, lib, TdfsError
 
 __all__ = [
-    "CSRGraph", "Plan", "Result", "Tuning", "TdfsError", "Device", "DeviceGraph",
-    "plan_pattern", "load_pattern", "reorder_pattern", "gen_pattern",
+    "CSRGraph", "Plan", "Result", "Tuning", "TdfsError", "Device", "DeviceGraph", "Batch",
+    "plan_pattern", "load_pattern", "reorder_pattern", "gen_pattern", "estimate_work",
 ]
 
 
@@ -116,6 +116,14 @@ class CSRGraph:
         return CSRGraph(self.rowptr, self.colidx, labels[: self.n].copy())
 
 
+def estimate_work(g: CSRGraph, plan: Plan, nsample: int = 2000, seed: int = 1, cap: float = 1e8) -> dict:
+    """Sampled DFS work estimate on a host CSR (tdfs_estimate_work)."""
+    out = (C.c_double * 3)()
+    check(lib().tdfs_estimate_work(g.n, g.nnz, i64p(g.rowptr), i32p(g.colidx), i32p(g.labels if plan.labeled else None),
+                                   C.byref(plan), nsample, seed, cap, out), "estimate")
+    return {"cost": out[0], "raw": out[1], "capped": bool(out[2])}
+
+
 def plan_pattern(text: str, labeled: bool) -> Plan:
     """Parses .g text and builds its plan (PatternPreprocessor, src/pattern.h:90-101)."""
     p = Plan()
@@ -157,6 +165,12 @@ class DeviceGraph:
         check(lib().tdfs_graph_info(self.handle, i64p(out)), "graph info")
         return {"n": int(out[0]), "nnz": int(out[1]), "ordered": bool(out[2]), "hubs": int(out[3])}
 
+    def estimate(self, plan: Plan, nsample: int = 2000, seed: int = 1, cap: float = 1e8) -> dict:
+        """Work estimate on the numbering the device counts this plan on."""
+        out = (C.c_double * 3)()
+        check(lib().tdfs_graph_estimate(self.handle, C.byref(plan), nsample, seed, cap, out), "graph estimate")
+        return {"cost": out[0], "raw": out[1], "capped": bool(out[2])}
+
     def order(self) -> np.ndarray | None:
         """New id of each vertex in the degree-ordered copy, or None."""
         if not self.info()["ordered"]:
@@ -173,6 +187,47 @@ class DeviceGraph:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Plans uploaded once; the launch sequence captured into a hipGraph on the
+    batch's own stream (tdfs_batch_create) and replayed on demand."""
+
+    def __init__(self, dev: "Device", dg: DeviceGraph, plans: list[Plan], blocks_per_cu: int = 0):
+        self.n = len(plans)
+        self._plans = (Plan * self.n)(*plans)
+        self.handle = C.c_void_p()
+        check(lib().tdfs_batch_create(dev.handle, dg.handle, self._plans, self.n, blocks_per_cu,
+                                      C.byref(self.handle)), "batch create")
+
+    def launch(self) -> None:
+        check(lib().tdfs_batch_launch(self.handle), "batch launch")
+
+    def done(self) -> bool:
+        rc = lib().tdfs_batch_done(self.handle)
+        if rc < 0:
+            check(rc, "batch done")
+        return rc == 1
+
+    def wait(self) -> list[Result]:
+        res = (Result * self.n)()
+        check(lib().tdfs_batch_wait(self.handle, res), "batch wait")
+        return list(res)
+
+    def run(self) -> list[Result]:
+        self.launch()
+        return self.wait()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().tdfs_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

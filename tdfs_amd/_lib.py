@@ -82,6 +82,10 @@ _SIGS = {
     "tdfs_gen_chung_lu": (C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_uint64, _I64P, _I32P]),
     "tdfs_gen_labels": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, _I32P]),
     "tdfs_gen_pattern": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_char_p, C.c_size_t]),
+    "tdfs_estimate_work": (C.c_int, [C.c_int32, C.c_int64, _I64P, _I32P, _I32P, C.POINTER(Plan), C.c_int32,
+                                     C.c_uint64, C.c_double, C.POINTER(C.c_double)]),
+    "tdfs_graph_estimate": (C.c_int, [_P, C.POINTER(Plan), C.c_int32, C.c_uint64, C.c_double,
+                                      C.POINTER(C.c_double)]),
     "tdfs_tuning_default": (None, [C.POINTER(Tuning)]),
     "tdfs_ctx_create": (C.c_int, [C.c_int32, C.POINTER(_P)]),
     "tdfs_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -96,6 +100,12 @@ _SIGS = {
     "tdfs_graph_order": (C.c_int, [_P, _I32P]),
     "tdfs_count": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int64, C.c_int64, C.POINTER(Result)]),
     "tdfs_count_batch": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int32, C.POINTER(Result)]),
+    "tdfs_batch_create": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "tdfs_batch_launch": (C.c_int, [_P]),
+    "tdfs_batch_done": (C.c_int, [_P]),
+    "tdfs_batch_wait": (C.c_int, [_P, C.POINTER(Result)]),
+    "tdfs_batch_run": (C.c_int, [_P, C.POINTER(Result)]),
+    "tdfs_batch_destroy": (None, [_P]),
 }
 
 _lib = None

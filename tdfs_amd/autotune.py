@@ -1,0 +1,100 @@
+"""Autotuner for the batched launch.
+
+Coordinate descent over the launch knobs of tdfs_tuning -- persistent-grid
+occupancy (blocks_per_cu: waves resident per CU, i.e. how many DFS stacks
+share a CU's registers and LDS), level-0 chunk (chunk_edges: edges per cursor
+grab), donation threshold (timeout_us) and donated piece size (split_chunk) --
+timing the hipGraph-replayed batch (median of `reps` replays per point; every
+replay's counts must equal the statistics launch).  The winner per workload is
+written to a JSON file that bench.py reads.
+
+    python -m tdfs_amd.autotune --workload c4 --out profiles/autotune.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import statistics
+import sys
+import time
+from pathlib import Path
+
+SPACE = {
+    "blocks_per_cu": [2, 3, 4],
+    "chunk_edges": [16, 32, 64, 128],
+    "timeout_us": [25, 100, 200, 1000],
+    "split_chunk": [1, 4],
+}
+
+
+def time_point(dev, dg, plans, counts, knobs: dict, reps: int) -> float:
+    import tdfs_amd as T
+
+    dev.set_tuning(**knobs)
+    b = T.Batch(dev, dg, plans)
+    ts = []
+    try:
+        for _ in range(reps):
+            res = b.run()
+            assert [int(r.count) for r in res] == counts, knobs
+            ts.append(res[0].kernel_ms)
+    finally:
+        b.close()
+    return statistics.median(ts)
+
+
+def tune(dev, dg, plans, reps: int = 3, rounds: int = 2, log=print) -> dict:
+    dev.set_stats(False)
+    counts = [int(r.count) for r in dev.count_batch(dg, plans)]
+    best = {"blocks_per_cu": 0, "chunk_edges": 64, "timeout_us": 200, "split_chunk": 1}
+    best_ms = time_point(dev, dg, plans, counts, best, reps)
+    trials = [{"knobs": dict(best), "ms": best_ms}]
+    log(f"start {best} {best_ms:.2f} ms")
+    for _ in range(rounds):
+        improved = False
+        for knob, values in SPACE.items():
+            for v in values:
+                if v == best[knob]:
+                    continue
+                cand = dict(best, **{knob: v})
+                ms = time_point(dev, dg, plans, counts, cand, reps)
+                trials.append({"knobs": cand, "ms": ms})
+                log(f"{cand} {ms:.2f} ms")
+                if ms < best_ms * 0.98:
+                    best, best_ms, improved = cand, ms, True
+        if not improved:
+            break
+    return {"best": best, "best_ms": best_ms, "trials": trials}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c4")
+    ap.add_argument("--out", default="profiles/autotune.json")
+    ap.add_argument("--reps", type=int, default=3)
+    args = ap.parse_args()
+    root = Path(__file__).resolve().parent.parent
+    sys.path.insert(0, str(root))
+    import bench
+    import tdfs_amd as T
+
+    g, pats, desc = bench.load_workload(args.workload)
+    dev = T.Device(0)
+    dg = dev.upload(g)
+    plans = [T.plan_pattern(t, lab) for t, lab in pats.values()]
+    t0 = time.time()
+    res = tune(dev, dg, plans, reps=args.reps, log=lambda s: print(s, flush=True))
+    res["workload"] = desc
+    res["seconds"] = round(time.time() - t0, 1)
+    out = Path(args.out)
+    data = json.loads(out.read_text()) if out.exists() else {}
+    data[args.workload] = res
+    out.write_text(json.dumps(data, indent=1))
+    print(json.dumps({"workload": args.workload, "best": res["best"], "best_ms": res["best_ms"]}))
+    dg.free()
+    dev.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

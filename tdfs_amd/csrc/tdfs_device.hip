@@ -1011,30 +1011,46 @@ struct tdfs_graph {
   GraphView view[2];
   int ordered;
   std::vector<int32_t> perm;  // new id of each vertex in view[1]
+  // host copies of both numberings (work estimates, serving admission)
+  std::vector<int64_t> h_rowptr[2];
+  std::vector<int32_t> h_colidx[2];
+  std::vector<int32_t> h_labels[2];
+};
+
+// Everything one launch needs, owned: stream, events, per-wave workspace,
+// task ring, control block, plan table, results, and pinned host copies of
+// the results.  A launch can be captured once into a hipGraph and replayed.
+struct Launch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int *wbuf = nullptr;
+  size_t wbuf_ints = 0;
+  TaskRec *q_data = nullptr;
+  unsigned long long *q_seq = nullptr;
+  unsigned long long q_cap = 0;
+  Ctrl *ctrl = nullptr;
+  unsigned long long *counts = nullptr, *stats = nullptr, *prof = nullptr;
+  DevPlan *plans = nullptr;
+  int plan_cap = 0;
+  // pinned host mirrors, written by the launch's D2H copies
+  unsigned long long *h_counts = nullptr, *h_stats = nullptr, *h_prof = nullptr;
+  Ctrl *h_ctrl = nullptr;
+  DevPlan *h_plans = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int grid = 0;
 };
 
 struct tdfs_ctx {
   int device;
-  hipStream_t stream;
-  hipEvent_t ev0, ev1;
   int cus;
   int wallclock_khz;
   int blocks_per_cu;        // occupancy limit of the timed kernel
   int blocks_per_cu_stats;  // and of the statistics kernel
   tdfs_tuning tune;
   int stats;
-  // buffers grown on demand
-  int *wbuf;
-  size_t wbuf_ints;
-  TaskRec *q_data;
-  unsigned long long *q_seq;
-  unsigned long long q_cap;
-  Ctrl *ctrl;
-  unsigned long long *counts;
-  unsigned long long *stats_buf;
-  DevPlan *plans;
-  int plan_cap;
-  unsigned long long *prof;
+  Launch main;              // used by tdfs_count / tdfs_count_batch
   unsigned long long last_prof[8];
 };
 
@@ -1079,6 +1095,41 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->degree_order <= 0) t->degree_order = d.degree_order;
 }
 
+static int launch_init(Launch &L, int device) {
+  L.device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&L.ev0));
+  HIP_TRY(hipEventCreate(&L.ev1));
+  HIP_TRY(hipMalloc(&L.ctrl, sizeof(Ctrl)));
+  HIP_TRY(hipMalloc(&L.prof, 8 * sizeof(unsigned long long)));
+  HIP_TRY(hipHostMalloc(&L.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(&L.h_prof, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  return TDFS_OK;
+}
+
+static void launch_free(Launch &L) {
+  if (L.stream) (void)hipStreamSynchronize(L.stream);
+  if (L.exec) (void)hipGraphExecDestroy(L.exec);
+  if (L.graph) (void)hipGraphDestroy(L.graph);
+  if (L.wbuf) (void)hipFree(L.wbuf);
+  if (L.q_data) (void)hipFree(L.q_data);
+  if (L.q_seq) (void)hipFree(L.q_seq);
+  if (L.ctrl) (void)hipFree(L.ctrl);
+  if (L.counts) (void)hipFree(L.counts);
+  if (L.stats) (void)hipFree(L.stats);
+  if (L.prof) (void)hipFree(L.prof);
+  if (L.plans) (void)hipFree(L.plans);
+  if (L.h_counts) (void)hipHostFree(L.h_counts);
+  if (L.h_stats) (void)hipHostFree(L.h_stats);
+  if (L.h_prof) (void)hipHostFree(L.h_prof);
+  if (L.h_ctrl) (void)hipHostFree(L.h_ctrl);
+  if (L.h_plans) (void)hipHostFree(L.h_plans);
+  if (L.ev0) (void)hipEventDestroy(L.ev0);
+  if (L.ev1) (void)hipEventDestroy(L.ev1);
+  if (L.stream) (void)hipStreamDestroy(L.stream);
+  L = Launch();
+}
+
 extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   if (!out) return fail(TDFS_E_ARG, "null out");
   int ndev = 0;
@@ -1086,8 +1137,9 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   if (device < 0 || device >= ndev) return fail(TDFS_E_ARG, "device %d out of range (%d devices)", device, ndev);
   HIP_TRY(hipSetDevice(device));
   tdfs_ctx *c = new tdfs_ctx();
-  std::memset(c, 0, sizeof(*c));
   c->device = device;
+  c->stats = 0;
+  std::memset(c->last_prof, 0, sizeof(c->last_prof));
   tdfs_tuning_default(&c->tune);
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
@@ -1095,17 +1147,17 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
   c->wallclock_khz = khz > 0 ? khz : 100000;
-  int nb = 0;
+  int nb = 0, nb2 = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdfs_match_kernel<false>, kBlock, 0));
-  int nb2 = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, tdfs_match_kernel<true>, kBlock, 0));
   c->blocks_per_cu = std::max(1, nb);
   c->blocks_per_cu_stats = std::max(1, nb2);
-  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreate(&c->ev0));
-  HIP_TRY(hipEventCreate(&c->ev1));
-  HIP_TRY(hipMalloc(&c->ctrl, sizeof(Ctrl)));
-  HIP_TRY(hipMalloc(&c->prof, 8 * sizeof(unsigned long long)));
+  int rc = launch_init(c->main, device);
+  if (rc) {
+    launch_free(c->main);
+    delete c;
+    return rc;
+  }
   *out = c;
   return TDFS_OK;
 }
@@ -1142,18 +1194,7 @@ extern "C" int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4) {
 extern "C" void tdfs_ctx_destroy(tdfs_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->wbuf) (void)hipFree(ctx->wbuf);
-  if (ctx->q_data) (void)hipFree(ctx->q_data);
-  if (ctx->q_seq) (void)hipFree(ctx->q_seq);
-  if (ctx->ctrl) (void)hipFree(ctx->ctrl);
-  if (ctx->prof) (void)hipFree(ctx->prof);
-  if (ctx->counts) (void)hipFree(ctx->counts);
-  if (ctx->stats_buf) (void)hipFree(ctx->stats_buf);
-  if (ctx->plans) (void)hipFree(ctx->plans);
-  (void)hipEventDestroy(ctx->ev0);
-  (void)hipEventDestroy(ctx->ev1);
-  (void)hipStreamDestroy(ctx->stream);
+  launch_free(ctx->main);
   delete ctx;
 }
 
@@ -1270,6 +1311,9 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
     delete g;
     return rc;
   }
+  g->h_rowptr[0].assign(rowptr, rowptr + n + 1);
+  g->h_colidx[0].assign(colidx, colidx + nnz);
+  if (labels) g->h_labels[0].assign(labels, labels + n);
   if (t.degree_order == 1 && n > 0) {
     // new id = rank by (degree, id)
     std::vector<int32_t> byrank(n);
@@ -1299,6 +1343,10 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
       return rc;
     }
     g->ordered = 1;
+    g->h_rowptr[1] = std::move(rp2);
+    ci2.resize((size_t)nnz);
+    g->h_colidx[1] = std::move(ci2);
+    g->h_labels[1] = std::move(lab2);
   }
   *out = g;
   return TDFS_OK;
@@ -1333,6 +1381,14 @@ extern "C" int tdfs_graph_order(const tdfs_graph *g, int32_t *perm_out) {
 // adjacent to vertex 1 or dP(2) <= dP(0); only then is the raw count
 // independent of the vertex numbering.
 static bool prune_is_sound(const tdfs_plan &p) { return p.adj[2][1] != 0 || p.deg[2] <= p.deg[0]; }
+
+extern "C" int tdfs_graph_estimate(const tdfs_graph *g, const tdfs_plan *plan, int32_t nsample, uint64_t seed,
+                                   double cap, double *out3) {
+  if (!g || !plan) return fail(TDFS_E_ARG, "null argument");
+  const int v = (g->ordered && prune_is_sound(*plan)) ? 1 : 0;
+  return tdfs::estimate(g->n, g->nnz, g->h_rowptr[v].data(), g->h_colidx[v].data(),
+                        g->labeled ? g->h_labels[v].data() : nullptr, plan, nsample, seed, cap, out3);
+}
 
 static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
   std::memset(&d, 0, sizeof(d));
@@ -1374,45 +1430,52 @@ static int check_plan(const tdfs_plan &p) {
   return TDFS_OK;
 }
 
-static int ensure_buffers(tdfs_ctx *ctx, const tdfs_graph *g, int nplans, long long waves, long long wstride) {
+static int launch_reserve(Launch &L, int nplans, long long waves, long long wstride, unsigned long long qcap) {
   const size_t need = (size_t)waves * (size_t)wstride;
-  if (need > ctx->wbuf_ints) {
-    if (ctx->wbuf) HIP_TRY(hipFree(ctx->wbuf));
-    ctx->wbuf = nullptr;
-    ctx->wbuf_ints = 0;
-    hipError_t e = hipMalloc(&ctx->wbuf, need * sizeof(int));
+  if (need > L.wbuf_ints) {
+    if (L.wbuf) HIP_TRY(hipFree(L.wbuf));
+    L.wbuf = nullptr;
+    L.wbuf_ints = 0;
+    hipError_t e = hipMalloc(&L.wbuf, need * sizeof(int));
     if (e != hipSuccess) return fail(TDFS_E_OOM, "workspace of %zu bytes: %s", need * sizeof(int), hipGetErrorString(e));
-    ctx->wbuf_ints = need;
+    L.wbuf_ints = need;
   }
-  const unsigned long long cap = 1ull << ctx->tune.queue_log2;
-  if (cap != ctx->q_cap) {
-    if (ctx->q_data) HIP_TRY(hipFree(ctx->q_data));
-    if (ctx->q_seq) HIP_TRY(hipFree(ctx->q_seq));
-    ctx->q_data = nullptr;
-    ctx->q_seq = nullptr;
-    HIP_TRY(hipMalloc(&ctx->q_data, sizeof(TaskRec) * cap));
-    HIP_TRY(hipMalloc(&ctx->q_seq, sizeof(unsigned long long) * cap));
-    ctx->q_cap = cap;
+  if (qcap != L.q_cap) {
+    if (L.q_data) HIP_TRY(hipFree(L.q_data));
+    if (L.q_seq) HIP_TRY(hipFree(L.q_seq));
+    L.q_data = nullptr;
+    L.q_seq = nullptr;
+    HIP_TRY(hipMalloc(&L.q_data, sizeof(TaskRec) * qcap));
+    HIP_TRY(hipMalloc(&L.q_seq, sizeof(unsigned long long) * qcap));
+    L.q_cap = qcap;
   }
-  if (nplans > ctx->plan_cap) {
-    if (ctx->plans) HIP_TRY(hipFree(ctx->plans));
-    if (ctx->counts) HIP_TRY(hipFree(ctx->counts));
-    if (ctx->stats_buf) HIP_TRY(hipFree(ctx->stats_buf));
-    ctx->plans = nullptr;
-    ctx->counts = nullptr;
-    ctx->stats_buf = nullptr;
-    HIP_TRY(hipMalloc(&ctx->plans, sizeof(DevPlan) * nplans));
-    HIP_TRY(hipMalloc(&ctx->counts, sizeof(unsigned long long) * nplans));
-    HIP_TRY(hipMalloc(&ctx->stats_buf, sizeof(unsigned long long) * 4 * nplans));
-    ctx->plan_cap = nplans;
+  if (nplans > L.plan_cap) {
+    if (L.plans) HIP_TRY(hipFree(L.plans));
+    if (L.counts) HIP_TRY(hipFree(L.counts));
+    if (L.stats) HIP_TRY(hipFree(L.stats));
+    if (L.h_counts) HIP_TRY(hipHostFree(L.h_counts));
+    if (L.h_stats) HIP_TRY(hipHostFree(L.h_stats));
+    if (L.h_plans) HIP_TRY(hipHostFree(L.h_plans));
+    L.plans = nullptr;
+    L.counts = L.stats = L.h_counts = L.h_stats = nullptr;
+    L.h_plans = nullptr;
+    L.plan_cap = 0;
+    HIP_TRY(hipMalloc(&L.plans, sizeof(DevPlan) * nplans));
+    HIP_TRY(hipMalloc(&L.counts, sizeof(unsigned long long) * nplans));
+    HIP_TRY(hipMalloc(&L.stats, sizeof(unsigned long long) * 4 * nplans));
+    HIP_TRY(hipHostMalloc(&L.h_counts, sizeof(unsigned long long) * nplans, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&L.h_stats, sizeof(unsigned long long) * 4 * nplans, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&L.h_plans, sizeof(DevPlan) * nplans, hipHostMallocDefault));
+    L.plan_cap = nplans;
   }
-  (void)g;
   return TDFS_OK;
 }
 
-static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int nplans, int64_t edge_begin,
-                     int64_t edge_end, tdfs_result *out) {
-  if (!ctx || !g || !plans || !out || nplans < 1) return fail(TDFS_E_ARG, "bad arguments");
+// Checks the plans, sizes and fills the launch's buffers and parameters.
+// blocks_cap > 0 limits the persistent grid to that many workgroups per CU.
+static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, const tdfs_plan *plans, int nplans,
+                          int64_t edge_begin, int64_t edge_end, bool stats, int blocks_cap, KParams &P) {
+  if (!ctx || !g || !plans || nplans < 1) return fail(TDFS_E_ARG, "bad arguments");
   if (g->device != ctx->device) return fail(TDFS_E_ARG, "graph and context on different devices");
   for (int i = 0; i < nplans; i++) {
     int rc = check_plan(plans[i]);
@@ -1420,31 +1483,31 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
     if (plans[i].labeled && !g->labeled) return fail(TDFS_E_ARG, "labeled plan on a graph uploaded without labels");
   }
   HIP_TRY(hipSetDevice(ctx->device));
-  fill_defaults(&ctx->tune);
+  tdfs_tuning t = ctx->tune;
+  fill_defaults(&t);
   if (edge_begin < 0) edge_begin = 0;
   if (edge_end < 0 || edge_end > g->nnz) edge_end = g->nnz;
   if (edge_begin > edge_end) edge_begin = edge_end;
-  const int chunk = ctx->tune.chunk_edges;
+  const int chunk = t.chunk_edges;
   const long long span = edge_end - edge_begin;
   const unsigned long long nchunks = (unsigned long long)((span + chunk - 1) / chunk);
   const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
   const long long wstride = 2ll * kChunkMax + (long long)(kMaxK - 3) * dstride;
-  const int occ = ctx->stats ? ctx->blocks_per_cu_stats : ctx->blocks_per_cu;
-  int bpc = ctx->tune.blocks_per_cu > 0 ? std::min(ctx->tune.blocks_per_cu, occ) : occ;
+  const int occ = stats ? ctx->blocks_per_cu_stats : ctx->blocks_per_cu;
+  int bpc = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, occ) : occ;
+  if (blocks_cap > 0) bpc = std::min(bpc, blocks_cap);
   const int grid = ctx->cus * bpc;
   const long long waves = (long long)grid * kWavesPerBlock;
-  int rc = ensure_buffers(ctx, g, nplans, waves, wstride);
+  int rc = launch_reserve(L, nplans, waves, wstride, 1ull << t.queue_log2);
   if (rc) return rc;
+  L.grid = grid;
+  for (int i = 0; i < nplans; i++) to_dev_plan(plans[i], L.h_plans[i]);
+  HIP_TRY(hipMemcpyAsync(L.plans, L.h_plans, sizeof(DevPlan) * nplans, hipMemcpyHostToDevice, L.stream));
 
-  std::vector<DevPlan> dp(nplans);
-  for (int i = 0; i < nplans; i++) to_dev_plan(plans[i], dp[i]);
-  HIP_TRY(hipMemcpyAsync(ctx->plans, dp.data(), sizeof(DevPlan) * nplans, hipMemcpyHostToDevice, ctx->stream));
-
-  KParams P;
-  std::memset(&P, 0, sizeof(P));
   bool all_sound = true;
   for (int i = 0; i < nplans; i++) all_sound = all_sound && prune_is_sound(plans[i]);
   const GraphView &V = (g->ordered && all_sound) ? g->view[1] : g->view[0];
+  std::memset(&P, 0, sizeof(P));
   P.rowptr = V.rowptr;
   P.colidx = V.colidx;
   P.degv = V.degv;
@@ -1454,61 +1517,67 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
   P.hub_idx = V.hub_idx;
   P.hub_bits = V.hub_bits;
   P.hub_words = V.hub_words;
-  P.plans = ctx->plans;
+  P.plans = L.plans;
   P.nplans = nplans;
   P.chunk = chunk;
   P.edge_lo = edge_begin;
   P.edge_hi = edge_end;
   P.nchunks = nchunks;
   P.total = (unsigned long long)nplans * nchunks * (unsigned long long)chunk;
-  P.wbuf = ctx->wbuf;
+  P.wbuf = L.wbuf;
   P.wstride = wstride;
   P.dstride = dstride;
-  P.q_data = ctx->q_data;
-  P.q_seq = ctx->q_seq;
-  P.q_mask = ctx->q_cap - 1;
-  P.counts = ctx->counts;
-  P.stats = ctx->stats_buf;
-  P.ctrl = ctx->ctrl;
-  P.prof = ctx->prof;
+  P.q_data = L.q_data;
+  P.q_seq = L.q_seq;
+  P.q_mask = L.q_cap - 1;
+  P.counts = L.counts;
+  P.stats = L.stats;
+  P.prof = L.prof;
+  P.ctrl = L.ctrl;
   const long long ticks_per_us = std::max(1, ctx->wallclock_khz / 1000);
-  P.timeout_ticks = ctx->tune.timeout_us < 0 ? -1 : (long long)ctx->tune.timeout_us * ticks_per_us;
+  P.timeout_ticks = t.timeout_us < 0 ? -1 : (long long)t.timeout_us * ticks_per_us;
   P.deadline_ticks = 300ll * 1000 * 1000 * ticks_per_us;  // 5 minutes
-  P.split_chunk = ctx->tune.split_chunk;
-  P.donate_levels = ctx->tune.donate_levels;
-  P.reuse = ctx->tune.reuse == 1 ? 1 : 0;
-  P.leaf_batch = ctx->tune.leaf_batch == 1 ? 1 : 0;
+  P.split_chunk = t.split_chunk;
+  P.donate_levels = t.donate_levels;
+  P.reuse = t.reuse == 1 ? 1 : 0;
+  P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
+  return TDFS_OK;
+}
 
-  const int init_blocks = (int)std::min<unsigned long long>(1024, (ctx->q_cap + 255) / 256);
-  hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->ctrl, ctx->counts,
-                     ctx->stats_buf, nplans, ctx->q_seq, ctx->q_cap, ctx->prof);
+// Enqueues reset + match kernel + result copies on the launch's stream.
+// No memset and no event nodes, so the sequence can be stream-captured.
+static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
+  const int init_blocks = (int)std::min<unsigned long long>(1024, (L.q_cap + 255) / 256);
+  hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, L.stream, L.ctrl, L.counts, L.stats,
+                     P.nplans, L.q_seq, L.q_cap, L.prof);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-  if (ctx->stats)
-    hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, P);
+  if (stats)
+    hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   else
-    hipLaunchKernelGGL(tdfs_match_kernel<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, P);
+    hipLaunchKernelGGL(tdfs_match_kernel<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-  std::vector<unsigned long long> counts(nplans), stats(4 * (size_t)nplans);
-  Ctrl ctrl;
-  HIP_TRY(hipMemcpyAsync(counts.data(), ctx->counts, sizeof(unsigned long long) * nplans, hipMemcpyDeviceToHost,
-                         ctx->stream));
-  HIP_TRY(hipMemcpyAsync(stats.data(), ctx->stats_buf, sizeof(unsigned long long) * 4 * nplans,
-                         hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(&ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->last_prof, ctx->prof, sizeof(ctx->last_prof), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  if (ctrl.error) return fail(TDFS_E_HIP, "device scheduler error %d (ring handshake or deadline)", ctrl.error);
+  HIP_TRY(hipMemcpyAsync(L.h_counts, L.counts, sizeof(unsigned long long) * P.nplans, hipMemcpyDeviceToHost,
+                         L.stream));
+  HIP_TRY(hipMemcpyAsync(L.h_stats, L.stats, sizeof(unsigned long long) * 4 * P.nplans, hipMemcpyDeviceToHost,
+                         L.stream));
+  HIP_TRY(hipMemcpyAsync(L.h_ctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.stream));
+  HIP_TRY(hipMemcpyAsync(L.h_prof, L.prof, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.stream));
+  return TDFS_OK;
+}
+
+// Reads the pinned result mirrors after the launch's stream has drained.
+static int launch_collect(const Launch &L, const tdfs_plan *plans, int nplans, double ms, tdfs_result *out,
+                          unsigned long long *prof_out) {
+  if (L.h_ctrl->error)
+    return fail(TDFS_E_HIP, "device scheduler error %d (ring handshake or deadline)", L.h_ctrl->error);
+  if (prof_out) std::memcpy(prof_out, L.h_prof, 8 * sizeof(unsigned long long));
   for (int i = 0; i < nplans; i++) {
     tdfs_result &r = out[i];
     std::memset(&r, 0, sizeof(r));
-    r.raw = counts[i];
-    r.alg_bytes = stats[4 * i + 0];
-    r.extensions = stats[4 * i + 1];
-    r.donated = stats[4 * i + 2];
+    r.raw = L.h_counts[i];
+    r.alg_bytes = L.h_stats[4 * i + 0];
+    r.extensions = L.h_stats[4 * i + 1];
+    r.donated = L.h_stats[4 * i + 2];
     r.kernel_ms = ms;
     if (plans[i].labeled) {
       r.count = r.raw;
@@ -1519,6 +1588,114 @@ static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans,
     }
   }
   return TDFS_OK;
+}
+
+static int run_plans(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int nplans, int64_t edge_begin,
+                     int64_t edge_end, tdfs_result *out) {
+  if (!out) return fail(TDFS_E_ARG, "null out");
+  Launch &L = ctx->main;
+  KParams P;
+  int rc = launch_prepare(ctx, L, g, plans, nplans, edge_begin, edge_end, ctx->stats != 0, 0, P);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(L.ev0, L.stream));
+  rc = launch_enqueue(L, P, ctx->stats != 0);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(L.ev1, L.stream));
+  HIP_TRY(hipStreamSynchronize(L.stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, L.ev0, L.ev1));
+  return launch_collect(L, plans, nplans, ms, out, ctx->stats ? ctx->last_prof : nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Prepared batches: plans uploaded once, the launch sequence captured into a
+// hipGraph on the batch's own stream, replayed by tdfs_batch_run.
+// ---------------------------------------------------------------------------
+struct tdfs_batch {
+  tdfs_ctx *ctx;
+  const tdfs_graph *g;
+  std::vector<tdfs_plan> plans;
+  Launch L;
+  bool stats;
+};
+
+extern "C" int tdfs_batch_create(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
+                                 int32_t blocks_per_cu, tdfs_batch **out) {
+  if (!ctx || !g || !plans || nplans < 1 || !out) return fail(TDFS_E_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  tdfs_batch *b = new tdfs_batch();
+  b->ctx = ctx;
+  b->g = g;
+  b->plans.assign(plans, plans + nplans);
+  b->stats = ctx->stats != 0;
+  int rc = launch_init(b->L, ctx->device);
+  KParams P;
+  if (!rc) rc = launch_prepare(ctx, b->L, g, plans, nplans, -1, -1, b->stats, blocks_per_cu, P);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(b->L.stream);  // plan upload done before capture
+    if (e) rc = fail(TDFS_E_HIP, "sync: %s", hipGetErrorString(e));
+  }
+  if (!rc) {
+    hipError_t e = hipStreamBeginCapture(b->L.stream, hipStreamCaptureModeThreadLocal);
+    if (e) rc = fail(TDFS_E_HIP, "begin capture: %s", hipGetErrorString(e));
+    if (!rc) {
+      const int erc = launch_enqueue(b->L, P, b->stats);
+      e = hipStreamEndCapture(b->L.stream, &b->L.graph);
+      if (erc) rc = erc;
+      else if (e) rc = fail(TDFS_E_HIP, "end capture: %s", hipGetErrorString(e));
+    }
+  }
+  if (!rc) {
+    hipError_t e = hipGraphInstantiate(&b->L.exec, b->L.graph, nullptr, nullptr, 0);
+    if (e) rc = fail(TDFS_E_HIP, "instantiate: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    launch_free(b->L);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return TDFS_OK;
+}
+
+// Starts one replay without waiting (results via tdfs_batch_wait).
+extern "C" int tdfs_batch_launch(tdfs_batch *b) {
+  if (!b) return fail(TDFS_E_ARG, "null batch");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipEventRecord(b->L.ev0, b->L.stream));
+  HIP_TRY(hipGraphLaunch(b->L.exec, b->L.stream));
+  HIP_TRY(hipEventRecord(b->L.ev1, b->L.stream));
+  return TDFS_OK;
+}
+
+// Returns 1 when the last replay has finished, 0 if still running.
+extern "C" int tdfs_batch_done(const tdfs_batch *b) {
+  if (!b) return fail(TDFS_E_ARG, "null batch");
+  const hipError_t e = hipEventQuery(b->L.ev1);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(TDFS_E_HIP, "event query: %s", hipGetErrorString(e));
+}
+
+extern "C" int tdfs_batch_wait(tdfs_batch *b, tdfs_result *out) {
+  if (!b || !out) return fail(TDFS_E_ARG, "null argument");
+  HIP_TRY(hipEventSynchronize(b->L.ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, b->L.ev0, b->L.ev1));
+  return launch_collect(b->L, b->plans.data(), (int)b->plans.size(), ms, out, nullptr);
+}
+
+extern "C" int tdfs_batch_run(tdfs_batch *b, tdfs_result *out) {
+  int rc = tdfs_batch_launch(b);
+  if (rc) return rc;
+  return tdfs_batch_wait(b, out);
+}
+
+extern "C" void tdfs_batch_destroy(tdfs_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  launch_free(b->L);
+  delete b;
 }
 
 extern "C" int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8) {

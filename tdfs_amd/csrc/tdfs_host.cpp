@@ -360,6 +360,136 @@ struct Alias {
 
 }  // namespace tdfs
 
+// ---------------------------------------------------------------------------
+// Work estimate: the DFS of the counting path (same level-0 filter, lists,
+// conditions and reuse) expanded from nsample uniformly drawn directed edges,
+// charging each extension the probe elements it tests times the lists it
+// tests them against.  Once the scaled estimate passes `cap` the expansion
+// stops and the estimate is marked capped (a lower bound: the pattern is then
+// treated as infeasible), so one call costs about cap * nsample / nnz work.
+// ---------------------------------------------------------------------------
+namespace {
+struct Estimator {
+  const int64_t *rowptr;
+  const int32_t *colidx;
+  const int32_t *labels;
+  const tdfs_plan *p;
+  int32_t M[TDFS_MAX_K];
+  std::vector<int32_t> cand[TDFS_MAX_K];
+  double cost = 0, raw = 0, cap = 0;
+  bool capped = false;
+
+  bool pass(int u, int32_t x) const {
+    if (!p->labeled) {
+      for (int j = 0; j < u; j++) {
+        const int op = p->cond[u][j];
+        if (op == TDFS_LT ? !(x < M[j]) : op == TDFS_GT ? !(x > M[j]) : x == M[j]) return false;
+      }
+      return true;
+    }
+    if (labels[x] != p->label[u]) return false;
+    for (int j = 0; j < u; j++)
+      if (M[j] == x) return false;
+    return true;
+  }
+
+  void dfs(int u) {
+    if (capped) return;
+    const int32_t *lp[TDFS_MAX_K];
+    int64_t ln[TDFS_MAX_K];
+    int nl = 0;
+    const int dep = p->shared_lvl[u];
+    if (dep >= 2) {
+      lp[nl] = cand[dep].data();
+      ln[nl++] = (int64_t)cand[dep].size();
+      for (int i = 0; i < p->num_bn_sh[u]; i++) {
+        const int32_t t = M[p->bn_sh[u][i]];
+        lp[nl] = colidx + rowptr[t];
+        ln[nl++] = rowptr[t + 1] - rowptr[t];
+      }
+    } else {
+      for (int i = 0; i < p->num_bn[u]; i++) {
+        const int32_t t = M[p->bn[u][i]];
+        lp[nl] = colidx + rowptr[t];
+        ln[nl++] = rowptr[t + 1] - rowptr[t];
+      }
+    }
+    int pi = 0;
+    for (int i = 1; i < nl; i++)
+      if (ln[i] < ln[pi]) pi = i;
+    cost += (double)ln[pi] * std::max(1, nl - 1) + nl;
+    std::vector<int32_t> &out = cand[u];
+    out.clear();
+    for (int64_t e = 0; e < ln[pi]; e++) {
+      const int32_t x = lp[pi][e];
+      if (!pass(u, x)) continue;
+      bool ok = true;
+      for (int i = 0; i < nl && ok; i++)
+        if (i != pi) ok = std::binary_search(lp[i], lp[i] + ln[i], x);
+      if (ok) out.push_back(x);
+    }
+    if (u == p->k - 1) {
+      raw += (double)out.size();
+      return;
+    }
+    if (cost > cap) {
+      capped = true;
+      return;
+    }
+    for (size_t i = 0; i < out.size() && !capped; i++) {
+      M[u] = out[i];
+      dfs(u + 1);
+    }
+  }
+};
+}  // namespace
+
+namespace tdfs {
+int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx, const int32_t *labels,
+             const tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3) {
+  if (!rowptr || !plan || !out3 || nsample < 1) return fail(TDFS_E_ARG, "bad estimate arguments");
+  if (plan->labeled && !labels) return fail(TDFS_E_ARG, "labeled plan needs labels");
+  out3[0] = out3[1] = out3[2] = 0;
+  if (nnz == 0) return TDFS_OK;
+  std::vector<int32_t> deg(n), maxnbr(n, -1);
+  for (int32_t v = 0; v < n; v++) deg[v] = (int32_t)(rowptr[v + 1] - rowptr[v]);
+  Rng r(seed);
+  Estimator E;
+  E.rowptr = rowptr;
+  E.colidx = colidx;
+  E.labels = labels;
+  E.p = plan;
+  const double scale = (double)nnz / (double)nsample;
+  E.cap = cap / scale;  // stop once the scaled estimate passes cap
+  for (int32_t s = 0; s < nsample && !E.capped; s++) {
+    const int64_t e = (int64_t)r.below((uint64_t)nnz);
+    const int32_t c = colidx[e];
+    const int32_t rr = (int32_t)(std::upper_bound(rowptr, rowptr + n + 1, e) - rowptr - 1);
+    E.cost += 2;
+    if (!plan->labeled) {
+      if (plan->sym01 && !(rr < c)) continue;
+    } else if (labels[rr] != plan->label[0] || labels[c] != plan->label[1]) {
+      continue;
+    }
+    if (deg[rr] < plan->deg[0] || deg[c] < plan->deg[1]) continue;
+    if (maxnbr[c] < 0) {
+      int m = 0;
+      for (int64_t q = rowptr[c]; q < rowptr[c + 1]; q++) m = std::max(m, deg[colidx[q]]);
+      maxnbr[c] = m;
+    }
+    if (maxnbr[c] < plan->deg[2]) continue;
+    E.M[0] = rr;
+    E.M[1] = c;
+    E.dfs(2);
+    if (E.cost > E.cap) E.capped = true;
+  }
+  out3[0] = E.cost * scale;
+  out3[1] = E.raw * scale;
+  out3[2] = E.capped ? 1.0 : 0.0;
+  return TDFS_OK;
+}
+}  // namespace tdfs
+
 using namespace tdfs;
 
 extern "C" {
@@ -625,3 +755,9 @@ int tdfs_gen_pattern(int32_t k, int32_t m, int32_t nlabels, uint64_t seed, char 
 }
 
 }  // extern "C"
+
+extern "C" int tdfs_estimate_work(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+                                  const int32_t *labels, const tdfs_plan *plan, int32_t nsample, uint64_t seed,
+                                  double cap, double *out3) {
+  return tdfs::estimate(n, nnz, rowptr, colidx, labels, plan, nsample, seed, cap, out3);
+}

@@ -3,10 +3,20 @@
 #ifndef TDFS_INTERNAL_H_
 #define TDFS_INTERNAL_H_
 
+#include <cstdint>
+
+struct tdfs_plan;
+
 namespace tdfs {
 // Records a formatted message for tdfs_last_error() and returns code.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 const char *last_error();
+
+// Sampled work estimate of one plan on a host CSR (see tdfs_estimate_work).
+// out3 = {estimated cost in list elements touched, estimated raw count,
+// 1 if some sampled subtree exceeded cap}.
+int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx, const int32_t *labels,
+             const struct tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3);
 }  // namespace tdfs
 
 #endif

@@ -1,0 +1,129 @@
+"""Seeded benchmark workloads (graphs + pattern sets) for the five configurations.
+
+c1  Erdos-Renyi G(n=20,000, m=100,000), seed 1; triangle.
+c2  Chung-Lu n=1.1M, m=3.0M, gamma=2.1, largest expected degree 30k, seed 2;
+    4-clique (data/pattern/2.g), diamond (1.g), 5-cycle (ordered along the cycle).
+c3  c2's graph + vertex labels uniform in {0..3} (seed 3); labeled house,
+    5-clique, 6-path, 6-cycle with labels drawn from seed 3.
+c4  c3's graph; 64 connected patterns from seed 4 (k = 3 + i mod 5, edge
+    count uniform in [k-1, k(k-1)/2], odd indices labeled with 4 labels), kept
+    only if the sampled work estimate stays under a budget (rule below).
+c5  Chung-Lu n=4M, m=34M, gamma=2.3, seed 5, labels uniform in {0..7};
+    a seeded query stream (k in 3..7, 30% labeled) under the same rule.
+
+Feasibility rule (c4, c5): the estimate of tdfs_estimate_work (2,000 edges
+sampled with seed 1, on the numbering the device counts on) must not be
+capped and must stay under `budget` list elements (c4: 1e10); a rejected
+draw is replaced by the next attempt (attempt counter in the seed) with the
+same vertex count, and after 50 rejected draws with one vertex fewer.  On
+the c3/c4 graph (its ~2k highest-degree vertices form a near-complete core)
+no 6- or 7-vertex draw fits 1e10: the 6-clique alone estimates 1.8e11 and
+the 7-clique 2.5e12 list elements.  Patterns whose
+count is known to be out of reach (c2's 5-cycle, c3's 6-path and 6-cycle:
+their estimates exceed 1e12 even labeled) are listed with their estimate and
+left out of the timed runs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import CSRGraph, estimate_work, gen_pattern, plan_pattern
+
+TRIANGLE = "v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n"
+K4 = "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n"
+DIAMOND = "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\n"
+K5 = "".join(f"v {i} 1\n" for i in range(5)) + "".join(
+    f"e {a} {b} 1\n" for a in range(5) for b in range(a + 1, 5))
+C5 = "".join(f"v {i} 1\n" for i in range(5)) + "".join(f"e {i} {(i + 1) % 5} 1\n" for i in range(4)) + "e 4 0 1\n"
+# house: square 0-1-2-3, roof 4 ~ {2, 3}
+HOUSE = ("".join(f"v {i} 1\n" for i in range(5))
+         + "e 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\ne 2 4 1\ne 3 4 1\n")
+P6 = "".join(f"v {i} 1\n" for i in range(6)) + "".join(f"e {i} {i + 1} 1\n" for i in range(5))
+C6 = P6 + "e 5 0 1\n"
+
+
+def with_pattern_labels(text: str, labels) -> str:
+    out, i = [], 0
+    for line in text.splitlines():
+        if line.startswith("v"):
+            _, v, _l = line.split()
+            out.append(f"v {v} {int(labels[i])}")
+            i += 1
+        else:
+            out.append(line)
+    return "\n".join(out) + "\n"
+
+
+def degree_order(g: CSRGraph) -> np.ndarray:
+    """The renumbering the device uses: new id = rank by (degree, id)."""
+    perm = np.empty(g.n, dtype=np.int64)
+    perm[np.argsort(g.degrees(), kind="stable")] = np.arange(g.n)
+    return perm
+
+
+def graph(name: str) -> CSRGraph:
+    if name == "c1":
+        return CSRGraph.erdos_renyi(20_000, 100_000, 1)
+    if name == "c2":
+        return CSRGraph.chung_lu(1_100_000, 3_000_000, 2.1, 30000.0, 2)
+    if name in ("c3", "c4"):
+        return CSRGraph.chung_lu(1_100_000, 3_000_000, 2.1, 30000.0, 2).with_labels(4, 3)
+    if name == "c5":
+        return CSRGraph.chung_lu(4_000_000, 34_000_000, 2.3, 60000.0, 5).with_labels(8, 5)
+    raise ValueError(name)
+
+
+def patterns(name: str) -> dict[str, tuple[str, bool]]:
+    """Fixed pattern sets: name -> (.g text, labeled)."""
+    if name == "c1":
+        return {"triangle": (TRIANGLE, False)}
+    if name == "c2":
+        return {"4-clique": (K4, False), "diamond": (DIAMOND, False)}
+    if name == "c3":
+        out = {}
+        for i, (nm, t) in enumerate((("house", HOUSE), ("5-clique", K5))):
+            k = sum(1 for l in t.splitlines() if l.startswith("v"))
+            labs = np.random.default_rng([3, i]).integers(0, 4, size=k)
+            out[nm] = (with_pattern_labels(t, labs), True)
+        return out
+    raise ValueError(name)
+
+
+# patterns of the configurations that no single GPU finishes (estimates
+# capped at 1e12 list elements); recorded, not timed
+OUT_OF_REACH = {"c2": {"5-cycle": (C5, False)}, "c3": {"6-path": (P6, True), "6-cycle": (C6, True)}}
+
+
+def random_patterns(g_device: CSRGraph, count: int, seed: int, labeled_every: int, nlabels: int,
+                    budget: float, nsample: int = 2000, max_attempts: int = 50) -> list[dict]:
+    """Seeded connected patterns kept by the feasibility rule: pattern i has
+    k = 3 + i mod 5 vertices; its edge count (uniform in [k-1, k(k-1)/2]),
+    edges and labels are redrawn (attempt counter in the seed) until the
+    estimate fits the budget; after every max_attempts rejected draws the
+    vertex count drops by one (never below 3).  g_device must be in the
+    numbering the device counts on (degree_order)."""
+    out = []
+    for i in range(count):
+        labeled = (i % labeled_every) == labeled_every - 1 if labeled_every > 1 else True
+        k_req = 3 + i % 5
+        k = k_req
+        attempt = 0
+        while True:
+            if attempt > 0 and attempt % max_attempts == 0 and k > 3:
+                k -= 1  # no draw of this size fits the budget: one vertex fewer
+            rng = np.random.default_rng([seed, i, attempt])
+            m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+            text = gen_pattern(k, m, nlabels if labeled else 0, int(rng.integers(1 << 62)))
+            plan = plan_pattern(text, labeled)
+            est = estimate_work(g_device, plan, nsample, 1, budget)
+            if not est["capped"] and est["cost"] <= budget:
+                out.append({"index": i, "attempt": attempt, "k_requested": k_req, "k": k, "m": m,
+                            "labeled": labeled, "text": text, "est_cost": est["cost"], "est_raw": est["raw"]})
+                break
+            attempt += 1
+    return out
+
+
+def c4_patterns(g_device: CSRGraph, budget: float = 1e10) -> list[dict]:
+    """64 patterns, seed 4, every other one labeled (4 labels)."""
+    return random_patterns(g_device, 64, 4, 2, 4, budget)

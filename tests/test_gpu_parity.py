@@ -201,6 +201,45 @@ def test_batch_equals_alone(dev):
         assert alone.alg_bytes == batch[j].alg_bytes, j
 
 
+def test_graph_replays_match_eager(dev):
+    """A captured batch replayed several times, with eager launches and host
+    synchronisation in between, returns the eager counts on every replay."""
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.1, 2000, 33).with_labels(4, 6)
+    dg = dev.upload(G)
+    texts = [fixture(i) for i in (1, 2, 4, 6, 9, 12)]
+    plans = [T.plan_pattern(t, i % 2) for i, t in enumerate(texts)]
+    dev.set_stats(False)
+    eager = [dev.count(dg, p).count for p in plans]
+    b = T.Batch(dev, dg, plans)
+    for rep in range(4):
+        say("graph replay", rep)
+        got = [r.count for r in b.run()]
+        assert got == eager, rep
+        assert dev.count(dg, plans[rep % len(plans)]).count == eager[rep % len(plans)]
+    b.close()
+
+
+def test_concurrent_batches(dev):
+    """Two batches sharing the device (capped grids, own streams) run at the
+    same time and each returns its counts alone."""
+    G = T.CSRGraph.chung_lu(30000, 150000, 2.1, 3000, 34).with_labels(3, 7)
+    dg = dev.upload(G)
+    pa = [T.plan_pattern(fixture(i), 0) for i in (1, 2, 3)]
+    pb = [T.plan_pattern(fixture(i), 1) for i in (8, 11, 13)]
+    want_a = [dev.count(dg, p).count for p in pa]
+    want_b = [dev.count(dg, p).count for p in pb]
+    ba = T.Batch(dev, dg, pa, blocks_per_cu=1)
+    bb = T.Batch(dev, dg, pb, blocks_per_cu=1)
+    for rep in range(3):
+        say("concurrent batches", rep)
+        ba.launch()
+        bb.launch()
+        assert [r.count for r in bb.wait()] == want_b
+        assert [r.count for r in ba.wait()] == want_a
+    ba.close()
+    bb.close()
+
+
 def test_edge_ranges_partition(dev):
     """Level-0 shards (directed-edge ranges) add up to the whole count."""
     G = T.CSRGraph.chung_lu(50000, 250000, 2.1, 4000, 41)
