@@ -144,6 +144,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (a kernel of known traffic "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: T-DFS shards level-0 edges across GPUs with a single "
@@ -157,6 +160,18 @@ def main() -> int:
     tfile = ROOT / "profiles" / "autotune.json"
     if tfile.exists():
         tuned = json.loads(tfile.read_text()).get(args.workload, {}).get("best", {})
+
+    if args.copy_calibration > 0:
+        import torch
+
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy calibration: {n} bytes read, {n} bytes written", file=sys.stderr)
+        del src, dst
 
     g, pats, desc = load_workload(args.workload)
     dev = T.Device(0)

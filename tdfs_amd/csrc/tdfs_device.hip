@@ -43,6 +43,7 @@ constexpr int kWavesPerBlock = 4;    // 256-thread workgroups
 constexpr int kBlock = kWavesPerBlock * kWave;
 constexpr long long kSpinMax = 1ll << 24;  // bound on waits for a slot handshake
 constexpr int kLeafPairCap = 4096;         // (owner, candidate) pairs per leaf batch
+constexpr int kMaxSegLabels = 64;          // label segments are built for at most this many labels
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
 struct DevPlan {
@@ -95,6 +96,9 @@ struct KParams {
   const int *src;
   const int *maxnbr;
   const unsigned char *labels;
+  const int *colidx_lab;      // each list re-sorted by (label, id); null when unlabeled
+  const int *loff;            // per vertex: nlab+1 offsets of its label segments
+  int nlab;
   const int *hub_idx;         // per vertex: row of its neighbour bitmap, or -1
   const unsigned *hub_bits;   // hub_words 32-bit words per hub
   long long hub_words;
@@ -203,6 +207,23 @@ __device__ __forceinline__ const unsigned *hub_row(const KParams &P, int v) {
   return h >= 0 ? P.hub_bits + (long long)h * P.hub_words : nullptr;
 }
 
+// Adjacency of t, restricted to its label-`lab` segment when lab >= 0 (the
+// segment is sorted by id, so it serves both as a probe and for membership).
+__device__ __forceinline__ const int *adj_list(const KParams &P, int t, int lab, int &len) {
+  if (lab >= 0) {
+    if (lab >= P.nlab) {  // no vertex carries this label
+      len = 0;
+      return P.colidx_lab;
+    }
+    const int *o = P.loff + (long long)t * (P.nlab + 1);
+    const int a = o[lab];
+    len = o[lab + 1] - a;
+    return P.colidx_lab + P.rowptr[t] + a;
+  }
+  len = P.degv[t];
+  return P.colidx + P.rowptr[t];
+}
+
 // ---------------------------------------------------------------------------
 // Candidate generation for pattern vertex u (2 <= u <= k-1) given M(0..u-1).
 // Returns |C(u)|; writes C(u) (ascending) to out unless out == nullptr.
@@ -224,19 +245,28 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   }
   int nl = 0;
   unsigned long long bytes = 0;
+  // labeled plans read only the label-l(u) segment of each adjacency list
+  const int seg = (pl->labeled && P.colidx_lab) ? pl->label[u] : -1;
+  int minfull = INT_MAX;  // statistics: shortest full input list
   if (dep >= 0) {
     lp[0] = lbase + (long long)(dep - 2) * P.dstride;  // C(dep) is the list of level dep-1
     ln[0] = S.len[dep - 1];
-    bytes += 4ull * (unsigned)ln[0];
+    if (STATS) {
+      bytes += 4ull * (unsigned)ln[0];
+      minfull = ln[0];
+    }
     const int nsh = pl->num_bn_sh[u];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
       if (i < nsh) {
         const int t = S.M[pl->bn_sh[u][i]];
-        lp[i + 1] = P.colidx + P.rowptr[t];
-        ln[i + 1] = P.degv[t];
+        lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
         hb[i + 1] = hub_row(P, t);
-        bytes += 4ull * (unsigned)ln[i + 1] + 16;
+        if (STATS) {
+          const int d = P.degv[t];
+          bytes += 4ull * (unsigned)d + 16;
+          minfull = min(minfull, d);
+        }
       }
     }
     nl = 1 + nsh;
@@ -246,10 +276,13 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
     for (int i = 0; i < 6; i++) {
       if (i < nb) {
         const int t = S.M[pl->bn[u][i]];
-        lp[i] = P.colidx + P.rowptr[t];
-        ln[i] = P.degv[t];
+        lp[i] = adj_list(P, t, seg, ln[i]);
         hb[i] = hub_row(P, t);
-        bytes += 4ull * (unsigned)ln[i] + 16;
+        if (STATS) {
+          const int d = P.degv[t];
+          bytes += 4ull * (unsigned)d + 16;
+          minfull = min(minfull, d);
+        }
       }
     }
     nl = nb;
@@ -269,8 +302,9 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   const bool leaf = (u == pl->k - 1);
   const bool labeled = pl->labeled != 0;
   if (STATS && count_stats) {
-    if (labeled) bytes += (unsigned)pn;
+    if (labeled) bytes += (unsigned)minfull;
   }
+  const bool label_check = labeled && seg < 0;
   // filter bounds
   int lo = -1, hi = INT_MAX;
   int nev[6];
@@ -319,7 +353,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
 #pragma unroll
           for (int j = 0; j < 6; j++)
             if ((nemask >> j) & 1) ok = ok && (x != nev[j]);
-          if (labeled && ok) ok = (int)P.labels[x] == lab;
+          if (label_check && ok) ok = (int)P.labels[x] == lab;
 #pragma unroll
           for (int i = 0; i < 7; i++)
             if (i < nl && i != pidx && ok) ok = hb[i] ? bit_test(hb[i], x) : contains(lp[i], ln[i], x);
@@ -396,19 +430,27 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   }
   int nl = 0;
   unsigned long long bytes = 0;
+  const int seg = (labeled && P.colidx_lab) ? pl->label[u] : -1;
+  int minfull = INT_MAX;
   if (dep >= 0) {
     lp[0] = lbase + (long long)(dep - 2) * P.dstride;
     ln[0] = S.len[dep - 1];
-    bytes += 4ull * (unsigned)ln[0];
+    if (STATS) {
+      bytes += 4ull * (unsigned)ln[0];
+      minfull = ln[0];
+    }
     const int nsh = pl->num_bn_sh[u];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
       if (i < nsh) {
         const int t = pick(pl->bn_sh[u][i]);
-        lp[i + 1] = P.colidx + P.rowptr[t];
-        ln[i + 1] = P.degv[t];
+        lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
         hb[i + 1] = hub_row(P, t);
-        bytes += 4ull * (unsigned)ln[i + 1] + 16;
+        if (STATS) {
+          const int d = P.degv[t];
+          bytes += 4ull * (unsigned)d + 16;
+          minfull = min(minfull, d);
+        }
       }
     }
     nl = 1 + nsh;
@@ -418,10 +460,13 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < 6; i++) {
       if (i < nb) {
         const int t = pick(pl->bn[u][i]);
-        lp[i] = P.colidx + P.rowptr[t];
-        ln[i] = P.degv[t];
+        lp[i] = adj_list(P, t, seg, ln[i]);
         hb[i] = hub_row(P, t);
-        bytes += 4ull * (unsigned)ln[i] + 16;
+        if (STATS) {
+          const int d = P.degv[t];
+          bytes += 4ull * (unsigned)d + 16;
+          minfull = min(minfull, d);
+        }
       }
     }
     nl = nb;
@@ -437,7 +482,8 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
       pidx = i;
     }
   }
-  if (STATS && labeled) bytes += (unsigned)pn;
+  if (STATS && labeled) bytes += (unsigned)minfull;
+  const bool label_check = labeled && seg < 0;
   int lo = -1, hi = INT_MAX;
   int nev[6];
   int nemask = 0;
@@ -530,7 +576,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
         ok = ok && (y != ne);
       }
     }
-    if (labeled && ok) ok = (int)P.labels[y] == lab;
+    if (label_check && ok) ok = (int)P.labels[y] == lab;
 #pragma unroll
     for (int i = 0; i < 7; i++) {
       if (i < nl) {
@@ -997,6 +1043,9 @@ struct GraphView {
   unsigned *hub_bits = nullptr;
   long long hub_words = 0;
   int nhubs = 0;
+  int *colidx_lab = nullptr;  // lists re-sorted by (label, id)
+  int *loff = nullptr;        // n * (nlab + 1) segment offsets
+  int nlab = 0;
 };
 
 struct tdfs_graph {
@@ -1207,6 +1256,8 @@ static void free_view(GraphView &v) {
   if (v.labels) (void)hipFree(v.labels);
   if (v.hub_idx) (void)hipFree(v.hub_idx);
   if (v.hub_bits) (void)hipFree(v.hub_bits);
+  if (v.colidx_lab) (void)hipFree(v.colidx_lab);
+  if (v.loff) (void)hipFree(v.loff);
   v = GraphView();
 }
 
@@ -1251,6 +1302,29 @@ static int upload_view(const tdfs_tuning &t, int32_t n, int64_t nnz, const int64
     if ((e = hipMemcpy(g.maxnbr, maxnbr.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
       return bail("copy maxnbr", e);
     if ((e = hipMemcpy(g.labels, lab8.data(), (size_t)n, hipMemcpyHostToDevice))) return bail("copy labels", e);
+  }
+  // Label segments: every list re-sorted by (label, id) with per-vertex
+  // offsets, so a labeled extension reads only the vertices of its label.
+  if (labels && n > 0) {
+    int nlab = 0;
+    for (int32_t v = 0; v < n; v++) nlab = std::max(nlab, (int)lab8[v] + 1);
+    if (nlab <= kMaxSegLabels) {
+      const int W = nlab + 1;
+      std::vector<int> off((size_t)n * W, 0), cl(nnz > 0 ? nnz : 1);
+      for (int32_t v = 0; v < n; v++) {
+        int *o = off.data() + (size_t)v * W;
+        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) o[lab8[colidx[e]] + 1]++;
+        for (int l = 0; l < nlab; l++) o[l + 1] += o[l];
+        std::vector<int> fill(o, o + nlab);
+        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) cl[rowptr[v] + fill[lab8[colidx[e]]]++] = colidx[e];
+      }
+      if ((e = hipMalloc(&g.colidx_lab, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("colidx_lab", e);
+      if ((e = hipMalloc(&g.loff, sizeof(int) * off.size()))) return bail("loff", e);
+      if (nnz > 0 && (e = hipMemcpy(g.colidx_lab, cl.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice)))
+        return bail("copy colidx_lab", e);
+      if ((e = hipMemcpy(g.loff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice))) return bail("copy loff", e);
+      g.nlab = nlab;
+    }
   }
   // Dense neighbour bitmaps for the highest-degree vertices, within a byte
   // budget: membership in a hub's list becomes one load.
@@ -1501,12 +1575,12 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   int rc = launch_reserve(L, nplans, waves, wstride, 1ull << t.queue_log2);
   if (rc) return rc;
   L.grid = grid;
-  for (int i = 0; i < nplans; i++) to_dev_plan(plans[i], L.h_plans[i]);
-  HIP_TRY(hipMemcpyAsync(L.plans, L.h_plans, sizeof(DevPlan) * nplans, hipMemcpyHostToDevice, L.stream));
-
   bool all_sound = true;
   for (int i = 0; i < nplans; i++) all_sound = all_sound && prune_is_sound(plans[i]);
   const GraphView &V = (g->ordered && all_sound) ? g->view[1] : g->view[0];
+  for (int i = 0; i < nplans; i++) to_dev_plan(plans[i], L.h_plans[i]);
+  HIP_TRY(hipMemcpyAsync(L.plans, L.h_plans, sizeof(DevPlan) * nplans, hipMemcpyHostToDevice, L.stream));
+
   std::memset(&P, 0, sizeof(P));
   P.rowptr = V.rowptr;
   P.colidx = V.colidx;
@@ -1517,6 +1591,9 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.hub_idx = V.hub_idx;
   P.hub_bits = V.hub_bits;
   P.hub_words = V.hub_words;
+  P.colidx_lab = V.colidx_lab;
+  P.loff = V.loff;
+  P.nlab = V.nlab;
   P.plans = L.plans;
   P.nplans = nplans;
   P.chunk = chunk;
