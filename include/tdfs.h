@@ -214,6 +214,26 @@ int tdfs_batch_wait(tdfs_batch *b, tdfs_result *out); /* wait, then results per 
 int tdfs_batch_run(tdfs_batch *b, tdfs_result *out);  /* launch + wait */
 void tdfs_batch_destroy(tdfs_batch *b);
 
+/* ---------------- serving (continuous batching) ---------------- */
+typedef struct tdfs_server tdfs_server;
+/* `slots` concurrent launches over the resident graph, each on its own
+ * stream with its own workspace and a persistent grid of blocks_per_cu
+ * workgroups per CU; up to max_batch waiting queries start together on an idle
+ * slot.  budget > 0: queries whose sampled work estimate exceeds it are
+ * refused with TDFS_E_BUDGET.  Device memory is fixed at creation. */
+int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slots, int32_t blocks_per_cu,
+                      int32_t max_batch, double budget, tdfs_server **out);
+/* Queues one query; returns its ticket. */
+int tdfs_serve_submit(tdfs_server *sv, const tdfs_plan *plan, uint64_t *ticket);
+/* Non-blocking progress: reports finished queries (ticket, result, latency
+ * from submit to collection in seconds; at most cap) and starts idle slots on
+ * waiting queries.  Call repeatedly. */
+int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *results, double *latency_s, int32_t cap,
+                    int32_t *nout);
+/* out4 = {queries waiting, slots busy, slots, queries running} */
+int tdfs_serve_status(const tdfs_server *sv, int64_t *out4);
+void tdfs_serve_destroy(tdfs_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

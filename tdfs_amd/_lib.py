@@ -106,6 +106,12 @@ _SIGS = {
     "tdfs_batch_wait": (C.c_int, [_P, C.POINTER(Result)]),
     "tdfs_batch_run": (C.c_int, [_P, C.POINTER(Result)]),
     "tdfs_batch_destroy": (None, [_P]),
+    "tdfs_serve_create": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.POINTER(_P)]),
+    "tdfs_serve_submit": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(C.c_uint64)]),
+    "tdfs_serve_poll": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(Result), C.POINTER(C.c_double), C.c_int32,
+                                  C.POINTER(C.c_int32)]),
+    "tdfs_serve_status": (C.c_int, [_P, _I64P]),
+    "tdfs_serve_destroy": (None, [_P]),
 }
 
 _lib = None

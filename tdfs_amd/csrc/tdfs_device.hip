@@ -24,9 +24,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <vector>
 
 #include "tdfs.h"
@@ -1773,6 +1775,164 @@ extern "C" void tdfs_batch_destroy(tdfs_batch *b) {
   (void)hipSetDevice(b->ctx->device);
   launch_free(b->L);
   delete b;
+}
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching over a device-resident graph.  The server owns
+// `slots` launches, each with its own stream, workspace, ring and result
+// buffers, and a persistent grid capped at blocks_per_cu workgroups per CU so
+// that the slots' kernels run side by side.  Submitted queries wait in a FIFO;
+// every poll collects the slots whose launch has finished and starts each idle
+// slot on up to max_batch waiting queries.  Admission: a query whose sampled
+// work estimate exceeds the budget (or is capped) is refused; the device memory
+// of all slots is fixed at creation.
+// ---------------------------------------------------------------------------
+struct ServeQuery {
+  uint64_t ticket;
+  tdfs_plan plan;
+  double t_submit;
+};
+
+struct ServeSlot {
+  Launch L;
+  std::vector<ServeQuery> running;
+  bool busy = false;
+};
+
+struct tdfs_server {
+  tdfs_ctx *ctx;
+  const tdfs_graph *g;
+  int blocks_per_cu;
+  int max_batch;
+  double budget;
+  int32_t est_samples;
+  std::vector<ServeSlot> slots;
+  std::deque<ServeQuery> waiting;
+  uint64_t next_ticket = 1;
+};
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+extern "C" int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slots, int32_t blocks_per_cu,
+                                 int32_t max_batch, double budget, tdfs_server **out) {
+  if (!ctx || !g || !out || slots < 1 || slots > 64 || max_batch < 1) return fail(TDFS_E_ARG, "bad server arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  tdfs_server *sv = new tdfs_server();
+  sv->ctx = ctx;
+  sv->g = g;
+  sv->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 1;
+  sv->max_batch = max_batch;
+  sv->budget = budget;
+  sv->est_samples = 1000;
+  sv->slots.resize(slots);
+  for (auto &sl : sv->slots) {
+    int rc = launch_init(sl.L, ctx->device);
+    if (rc) {
+      for (auto &x : sv->slots) launch_free(x.L);
+      delete sv;
+      return rc;
+    }
+  }
+  *out = sv;
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_serve_submit(tdfs_server *sv, const tdfs_plan *plan, uint64_t *ticket) {
+  if (!sv || !plan || !ticket) return fail(TDFS_E_ARG, "null argument");
+  int rc = check_plan(*plan);
+  if (rc) return rc;
+  if (plan->labeled && !sv->g->labeled) return fail(TDFS_E_ARG, "labeled query on an unlabeled graph");
+  if (sv->budget > 0) {
+    double est[3];
+    rc = tdfs_graph_estimate(sv->g, plan, sv->est_samples, 1, sv->budget, est);
+    if (rc) return rc;
+    if (est[2] != 0.0 || est[0] > sv->budget)
+      return fail(TDFS_E_BUDGET, "query refused: estimated work %.3g exceeds the budget %.3g", est[0], sv->budget);
+  }
+  ServeQuery q;
+  q.ticket = sv->next_ticket++;
+  q.plan = *plan;
+  q.t_submit = now_s();
+  sv->waiting.push_back(q);
+  *ticket = q.ticket;
+  return TDFS_OK;
+}
+
+// Collects finished slots into (tickets, results, latency_s) up to cap and
+// starts idle slots on waiting queries.  *nout = entries written.
+extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *results, double *latency_s,
+                               int32_t cap, int32_t *nout) {
+  if (!sv || !nout || (cap > 0 && (!tickets || !results))) return fail(TDFS_E_ARG, "null argument");
+  HIP_TRY(hipSetDevice(sv->ctx->device));
+  int n = 0;
+  for (auto &sl : sv->slots) {
+    if (!sl.busy) continue;
+    if ((int)sl.running.size() > cap - n) continue;  // no room to report this slot yet
+    const hipError_t e = hipEventQuery(sl.L.ev1);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return fail(TDFS_E_HIP, "event query: %s", hipGetErrorString(e));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, sl.L.ev0, sl.L.ev1));
+    std::vector<tdfs_plan> plans;
+    for (auto &q : sl.running) plans.push_back(q.plan);
+    std::vector<tdfs_result> res(plans.size());
+    const double t = now_s();
+    int rc = launch_collect(sl.L, plans.data(), (int)plans.size(), ms, res.data(), nullptr);
+    if (rc) return rc;
+    for (size_t i = 0; i < plans.size(); i++) {
+      tickets[n] = sl.running[i].ticket;
+      results[n] = res[i];
+      if (latency_s) latency_s[n] = t - sl.running[i].t_submit;
+      n++;
+    }
+    sl.running.clear();
+    sl.busy = false;
+  }
+  for (auto &sl : sv->slots) {
+    if (sl.busy || sv->waiting.empty()) continue;
+    std::vector<tdfs_plan> plans;
+    while (!sv->waiting.empty() && (int)sl.running.size() < sv->max_batch) {
+      sl.running.push_back(sv->waiting.front());
+      plans.push_back(sv->waiting.front().plan);
+      sv->waiting.pop_front();
+    }
+    KParams P;
+    int rc = launch_prepare(sv->ctx, sl.L, sv->g, plans.data(), (int)plans.size(), -1, -1, false, sv->blocks_per_cu, P);
+    if (!rc) {
+      HIP_TRY(hipEventRecord(sl.L.ev0, sl.L.stream));
+      rc = launch_enqueue(sl.L, P, false);
+      if (!rc) HIP_TRY(hipEventRecord(sl.L.ev1, sl.L.stream));
+    }
+    if (rc) return rc;
+    sl.busy = true;
+  }
+  *nout = n;
+  return TDFS_OK;
+}
+
+// out4 = {queries waiting, slots busy, slots, queries running}
+extern "C" int tdfs_serve_status(const tdfs_server *sv, int64_t *out4) {
+  if (!sv || !out4) return fail(TDFS_E_ARG, "null argument");
+  int busy = 0;
+  int64_t running = 0;
+  for (const auto &sl : sv->slots) {
+    busy += sl.busy;
+    running += (int64_t)sl.running.size();
+  }
+  out4[0] = (int64_t)sv->waiting.size();
+  out4[1] = busy;
+  out4[2] = (int64_t)sv->slots.size();
+  out4[3] = running;
+  return TDFS_OK;
+}
+
+extern "C" void tdfs_serve_destroy(tdfs_server *sv) {
+  if (!sv) return;
+  (void)hipSetDevice(sv->ctx->device);
+  for (auto &sl : sv->slots) launch_free(sl.L);
+  delete sv;
 }
 
 extern "C" int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8) {

@@ -240,6 +240,36 @@ def test_concurrent_batches(dev):
     bb.close()
 
 
+def test_serving_mix_equals_alone(dev):
+    """Queries streamed through the continuous-batching server (several slots
+    running side by side, queries arriving while others run) return exactly
+    their alone counts."""
+    from tdfs_amd import serve
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.2, 1500, 61).with_labels(4, 3)
+    dg = dev.upload(G)
+    qs = serve.query_stream(dg, 40, 9, 0.4, 4, 5e8)
+    plans = [T.plan_pattern(q["text"], q["labeled"]) for q in qs]
+    alone = [dev.count(dg, p).count for p in plans]
+    sv = serve.Server(dev, dg, slots=3, blocks_per_cu=1, max_batch=8, budget=5e8)
+    for rate in (200.0, 5000.0):
+        say("serving stream", rate)
+        r = serve.run_stream(sv, plans, rate, 7)
+        assert r["counts"] == alone, rate
+        assert r["p99_ms"] >= r["p50_ms"] > 0
+    sv.close()
+
+
+def test_serving_refuses_over_budget(dev):
+    from tdfs_amd import serve
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.1, 2000, 62)
+    dg = dev.upload(G)
+    sv = serve.Server(dev, dg, slots=1, blocks_per_cu=1, max_batch=4, budget=1e3)
+    with pytest.raises(T.TdfsError) as e:
+        sv.submit(T.plan_pattern(fixture(7), 0))
+    assert e.value.code == -7
+    sv.close()
+
+
 def test_edge_ranges_partition(dev):
     """Level-0 shards (directed-edge ranges) add up to the whole count."""
     G = T.CSRGraph.chung_lu(50000, 250000, 2.1, 4000, 41)
