@@ -1066,6 +1066,7 @@ struct tdfs_graph {
   std::vector<int64_t> h_rowptr[2];
   std::vector<int32_t> h_colidx[2];
   std::vector<int32_t> h_labels[2];
+  std::vector<int32_t> h_deg[2], h_maxnbr[2];
 };
 
 // Everything one launch needs, owned: stream, events, per-wave workspace,
@@ -1390,6 +1391,16 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
   g->h_rowptr[0].assign(rowptr, rowptr + n + 1);
   g->h_colidx[0].assign(colidx, colidx + nnz);
   if (labels) g->h_labels[0].assign(labels, labels + n);
+  auto derive = [&](int v) {
+    const std::vector<int64_t> &rp = g->h_rowptr[v];
+    const std::vector<int32_t> &ci = g->h_colidx[v];
+    g->h_deg[v].resize(n);
+    g->h_maxnbr[v].assign(n, -1);
+    for (int32_t x = 0; x < n; x++) g->h_deg[v][x] = (int32_t)(rp[x + 1] - rp[x]);
+    for (int32_t x = 0; x < n; x++)
+      for (int64_t e = rp[x]; e < rp[x + 1]; e++) g->h_maxnbr[v][x] = std::max(g->h_maxnbr[v][x], g->h_deg[v][ci[e]]);
+  };
+  derive(0);
   if (t.degree_order == 1 && n > 0) {
     // new id = rank by (degree, id)
     std::vector<int32_t> byrank(n);
@@ -1423,6 +1434,7 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
     ci2.resize((size_t)nnz);
     g->h_colidx[1] = std::move(ci2);
     g->h_labels[1] = std::move(lab2);
+    derive(1);
   }
   *out = g;
   return TDFS_OK;
@@ -1463,7 +1475,8 @@ extern "C" int tdfs_graph_estimate(const tdfs_graph *g, const tdfs_plan *plan, i
   if (!g || !plan) return fail(TDFS_E_ARG, "null argument");
   const int v = (g->ordered && prune_is_sound(*plan)) ? 1 : 0;
   return tdfs::estimate(g->n, g->nnz, g->h_rowptr[v].data(), g->h_colidx[v].data(),
-                        g->labeled ? g->h_labels[v].data() : nullptr, plan, nsample, seed, cap, out3);
+                        g->labeled ? g->h_labels[v].data() : nullptr, plan, nsample, seed, cap, out3,
+                        g->h_deg[v].data(), g->h_maxnbr[v].data());
 }
 
 static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {

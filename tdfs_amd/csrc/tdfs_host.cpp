@@ -446,13 +446,24 @@ struct Estimator {
 
 namespace tdfs {
 int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx, const int32_t *labels,
-             const tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3) {
+             const tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3,
+             const int32_t *deg_in, const int32_t *maxnbr_in) {
   if (!rowptr || !plan || !out3 || nsample < 1) return fail(TDFS_E_ARG, "bad estimate arguments");
   if (plan->labeled && !labels) return fail(TDFS_E_ARG, "labeled plan needs labels");
   out3[0] = out3[1] = out3[2] = 0;
   if (nnz == 0) return TDFS_OK;
-  std::vector<int32_t> deg(n), maxnbr(n, -1);
-  for (int32_t v = 0; v < n; v++) deg[v] = (int32_t)(rowptr[v + 1] - rowptr[v]);
+  auto degree = [&](int32_t v) { return deg_in ? deg_in[v] : (int32_t)(rowptr[v + 1] - rowptr[v]); };
+  std::vector<int32_t> maxnbr_local;
+  if (!maxnbr_in) maxnbr_local.assign(n, -1);
+  auto max_nbr_deg = [&](int32_t c) {
+    if (maxnbr_in) return maxnbr_in[c];
+    if (maxnbr_local[c] < 0) {
+      int m = 0;
+      for (int64_t q = rowptr[c]; q < rowptr[c + 1]; q++) m = std::max(m, degree(colidx[q]));
+      maxnbr_local[c] = m;
+    }
+    return maxnbr_local[c];
+  };
   Rng r(seed);
   Estimator E;
   E.rowptr = rowptr;
@@ -471,13 +482,8 @@ int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colid
     } else if (labels[rr] != plan->label[0] || labels[c] != plan->label[1]) {
       continue;
     }
-    if (deg[rr] < plan->deg[0] || deg[c] < plan->deg[1]) continue;
-    if (maxnbr[c] < 0) {
-      int m = 0;
-      for (int64_t q = rowptr[c]; q < rowptr[c + 1]; q++) m = std::max(m, deg[colidx[q]]);
-      maxnbr[c] = m;
-    }
-    if (maxnbr[c] < plan->deg[2]) continue;
+    if (degree(rr) < plan->deg[0] || degree(c) < plan->deg[1]) continue;
+    if (max_nbr_deg(c) < plan->deg[2]) continue;
     E.M[0] = rr;
     E.M[1] = c;
     E.dfs(2);

@@ -15,8 +15,11 @@ const char *last_error();
 // Sampled work estimate of one plan on a host CSR (see tdfs_estimate_work).
 // out3 = {estimated cost in list elements touched, estimated raw count,
 // 1 if some sampled subtree exceeded cap}.
+// deg / maxnbr (per vertex: degree, max neighbour degree) may be null; they
+// are then derived on the fly.
 int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx, const int32_t *labels,
-             const struct tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3);
+             const struct tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3,
+             const int32_t *deg = nullptr, const int32_t *maxnbr = nullptr);
 }  // namespace tdfs
 
 #endif

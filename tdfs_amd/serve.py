@@ -62,12 +62,14 @@ class Server:
             pass
 
 
-def query_stream(dg: DeviceGraph, count: int, seed: int, labeled_frac: float, nlabels: int, budget: float,
-                 nsample: int = 1000, max_attempts: int = 50) -> list[dict]:
+def query_stream(dg, count: int, seed: int, labeled_frac: float, nlabels: int, budget: float,
+                 nsample: int = 1000, max_attempts: int = 50, progress: bool = False) -> list[dict]:
     """Seeded queries: k uniform in 3..7, edge count uniform in [k-1, k(k-1)/2],
     labeled with probability labeled_frac (nlabels labels); a draw whose
-    sampled work estimate is capped or above budget is redrawn (after
-    max_attempts rejections, with one vertex fewer)."""
+    sampled work estimate is capped or above budget is redrawn: after every
+    max_attempts rejections with one vertex fewer, and once at 3 vertices
+    with labels.  `dg` is a DeviceGraph or anything with the same
+    estimate(plan, nsample, seed, cap) method."""
     out = []
     for i in range(count):
         rng0 = np.random.default_rng([seed, i])
@@ -75,17 +77,23 @@ def query_stream(dg: DeviceGraph, count: int, seed: int, labeled_frac: float, nl
         labeled = bool(rng0.random() < labeled_frac)
         attempt = 0
         while True:
-            if attempt > 0 and attempt % max_attempts == 0 and k > 3:
-                k -= 1
+            if attempt > 0 and attempt % max_attempts == 0:
+                if k > 3:
+                    k -= 1
+                else:
+                    labeled = True
             rng = np.random.default_rng([seed, i, attempt + 1])
             m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
             text = gen_pattern(k, m, nlabels if labeled else 0, int(rng.integers(1 << 62)))
             plan = plan_pattern(text, labeled)
             est = dg.estimate(plan, nsample, 1, budget)
             if not est["capped"] and est["cost"] <= budget:
-                out.append({"index": i, "k": k, "m": m, "labeled": labeled, "text": text, "est_cost": est["cost"]})
+                out.append({"index": i, "k": k, "m": m, "labeled": labeled, "text": text, "est_cost": est["cost"],
+                            "attempts": attempt + 1})
                 break
             attempt += 1
+        if progress and (i + 1) % 25 == 0:
+            print(f"query_stream: {i + 1}/{count} queries drawn", file=sys.stderr, flush=True)
     return out
 
 
@@ -136,7 +144,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--blocks-per-cu", type=int, default=1)
     ap.add_argument("--max-batch", type=int, default=32)
-    ap.add_argument("--budget", type=float, default=2e9)
+    ap.add_argument("--budget", type=float, default=2e10)
     ap.add_argument("--check", type=int, default=20, help="queries recounted alone for equality")
     args = ap.parse_args()
     t = time.time()
@@ -145,7 +153,7 @@ def main() -> int:
     dev = Device(0)
     dg = dev.upload(g)
     print(f"uploaded {time.time() - t:.1f} s", file=sys.stderr, flush=True)
-    qs = query_stream(dg, args.queries, 5, 0.3, 8, args.budget)
+    qs = query_stream(dg, args.queries, 5, 0.3, 8, args.budget, progress=True)
     plans = [plan_pattern(q["text"], q["labeled"]) for q in qs]
     print(f"{len(plans)} queries {time.time() - t:.1f} s", file=sys.stderr, flush=True)
     out = {"workload": "c5: Chung-Lu n=4M m=34M gamma=2.3 seed 5, 8 labels; seeded query stream (k 3-7, 30% labeled)",

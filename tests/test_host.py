@@ -222,3 +222,21 @@ def test_c4_pattern_set_is_reproducible():
     D = G.relabeled(W.degree_order(G))
     again = W.random_patterns(D, 6, 4, 2, 4, 1e10)
     assert [q["text"] for q in again] == [p["text"] for p in pats[:6]]
+
+
+def test_query_stream_rule_terminates_and_is_seeded():
+    """The serving stream's feasibility rule always terminates (even when no
+    unlabeled 3-vertex draw fits the budget) and is reproducible."""
+    from tdfs_amd import serve, workloads as W
+    G = T.CSRGraph.chung_lu(30000, 200000, 2.2, 3000, 5).with_labels(8, 5)
+    D = G.relabeled(W.degree_order(G))
+
+    class HostEstimator:
+        def estimate(self, plan, nsample, seed, cap):
+            return T.estimate_work(D, plan, nsample, seed, cap)
+
+    tri = T.estimate_work(D, T.plan_pattern(W.TRIANGLE, False), 1000, 1, 1e15)["cost"]
+    a = serve.query_stream(HostEstimator(), 12, 5, 0.3, 8, tri / 4)  # too small for any unlabeled triangle
+    b = serve.query_stream(HostEstimator(), 12, 5, 0.3, 8, tri / 4)
+    assert [q["text"] for q in a] == [q["text"] for q in b]
+    assert all(q["est_cost"] <= tri / 4 for q in a)
