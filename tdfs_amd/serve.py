@@ -8,7 +8,8 @@ wait in a FIFO, and start in groups of up to `max_batch` on whichever of the
 `run_stream` replays a seeded Poisson arrival process against a server and
 reports p50/p99 latency (submit -> result), throughput and matches/s.
 
-    python -m tdfs_amd.serve --rates 50,200,1000,2000 --queries 200
+    python -m tdfs_amd.serve --rates 50,200,1000,2000 --queries 200 \
+        --queries-file tests/golden/c5_queries.json
 """
 from __future__ import annotations
 
@@ -97,7 +98,7 @@ def query_stream(dg, count: int, seed: int, labeled_frac: float, nlabels: int, b
     return out
 
 
-def run_stream(server: Server, plans: list[Plan], rate: float, seed: int) -> dict:
+def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progress: int = 0) -> dict:
     """Submits plans at seeded Poisson arrival times (rate queries/s), polls
     until all are answered; latency = submit -> collected."""
     rng = np.random.default_rng(seed)
@@ -116,6 +117,9 @@ def run_stream(server: Server, plans: list[Plan], rate: float, seed: int) -> dic
             i = ticket_of[tk]
             results[i] = r
             lat[i] = l
+            if progress and len(results) % progress == 0:
+                print(f"  rate {rate:g}: {len(results)}/{len(plans)} answered, "
+                      f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         if nxt < len(plans) and not server.status()["running"]:
             time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
@@ -146,6 +150,7 @@ def main() -> int:
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--budget", type=float, default=2e10)
     ap.add_argument("--check", type=int, default=20, help="queries recounted alone for equality")
+    ap.add_argument("--queries-file", default="", help="committed query list (json) instead of drawing one")
     args = ap.parse_args()
     t = time.time()
     g = W.graph("c5")
@@ -153,7 +158,11 @@ def main() -> int:
     dev = Device(0)
     dg = dev.upload(g)
     print(f"uploaded {time.time() - t:.1f} s", file=sys.stderr, flush=True)
-    qs = query_stream(dg, args.queries, 5, 0.3, 8, args.budget, progress=True)
+    if args.queries_file:
+        from pathlib import Path
+        qs = json.loads(Path(args.queries_file).read_text())["queries"][: args.queries]
+    else:
+        qs = query_stream(dg, args.queries, 5, 0.3, 8, args.budget, progress=True)
     plans = [plan_pattern(q["text"], q["labeled"]) for q in qs]
     print(f"{len(plans)} queries {time.time() - t:.1f} s", file=sys.stderr, flush=True)
     out = {"workload": "c5: Chung-Lu n=4M m=34M gamma=2.3 seed 5, 8 labels; seeded query stream (k 3-7, 30% labeled)",
@@ -162,9 +171,12 @@ def main() -> int:
     server = Server(dev, dg, args.slots, args.blocks_per_cu, args.max_batch, args.budget)
     alone = {}
     for i in range(min(args.check, len(plans))):
-        alone[i] = int(dev.count(dg, plans[i]).count)
+        r = dev.count(dg, plans[i])
+        alone[i] = int(r.count)
+        print(f"alone {i}: k={plans[i].k} labeled={plans[i].labeled} count={r.count} {r.kernel_ms:.2f} ms",
+              file=sys.stderr, flush=True)
     for rate in [float(x) for x in args.rates.split(",")]:
-        r = run_stream(server, plans, rate, 50 + int(rate))
+        r = run_stream(server, plans, rate, 50 + int(rate), progress=25)
         same = all(r["counts"][i] == c for i, c in alone.items())
         r["alone_equal"] = same
         del r["counts"]
