@@ -88,7 +88,9 @@ typedef struct tdfs_tuning {
   int32_t hub_budget_mb;     /* bitmap memory per uploaded graph (default 4096; <0: none); read at upload */
   int32_t degree_order;      /* 1: also keep a copy renumbered by ascending degree and count on it whenever
                                 the result cannot depend on the numbering (default); 2: off.  Read at upload */
-  int32_t reserved[3];
+  int32_t time_limit_ms;     /* > 0: work still pending this long into a launch is dropped and the plans it
+                                belonged to are flagged TDFS_INCOMPLETE (default 0: no limit) */
+  int32_t reserved[2];
 } tdfs_tuning;
 
 /* Result of one counted pattern. */
@@ -99,7 +101,11 @@ typedef struct tdfs_result {
   uint64_t extensions;   /* DFS extensions performed (stats) */
   uint64_t donated;      /* tasks pushed to the task ring */
   double kernel_ms;      /* device time of the launch (hipEvents on the library stream) */
+  uint32_t flags;        /* TDFS_INCOMPLETE: the time limit dropped part of this plan's work */
+  uint32_t reserved;
 } tdfs_result;
+
+#define TDFS_INCOMPLETE 1u
 
 typedef struct tdfs_ctx tdfs_ctx;
 typedef struct tdfs_graph tdfs_graph;
@@ -149,8 +155,10 @@ int tdfs_gen_pattern(int32_t k, int32_t m, int32_t nlabels, uint64_t seed, char 
                      size_t cap);
 
 /* ---------------- host side: work estimate ---------------- */
-/* Expands the counting DFS from nsample uniformly drawn directed edges
- * (seeded) and scales by nnz / nsample.  out3 = {estimated cost in list
+/* Expands the counting DFS from nsample directed edges drawn (seeded)
+ * uniformly inside 16 equal edge-index strata, and scales each stratum by its
+ * size / samples (with rows sorted by degree, the hub rows form the last
+ * strata and are always sampled).  out3 = {estimated cost in list
  * elements tested, estimated raw count, 1 if the scaled cost passed cap and
  * the expansion stopped (then the cost is a lower bound)}.  Used for seeded
  * feasibility filters and serving admission. */

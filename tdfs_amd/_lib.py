@@ -49,7 +49,8 @@ class Tuning(C.Structure):
         ("hub_min_degree", C.c_int32),
         ("hub_budget_mb", C.c_int32),
         ("degree_order", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("time_limit_ms", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -61,6 +62,8 @@ class Result(C.Structure):
         ("extensions", C.c_uint64),
         ("donated", C.c_uint64),
         ("kernel_ms", C.c_double),
+        ("flags", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

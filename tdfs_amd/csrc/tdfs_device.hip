@@ -131,6 +131,7 @@ struct KParams {
   int donate_levels;
   int reuse;
   int leaf_batch;
+  long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
 };
 
 struct WaveLds {
@@ -781,7 +782,7 @@ template <bool STATS>
 __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
                     const int *L0r, const int *L0c, int *lbase, unsigned long long &cnt,
                     unsigned long long &balg, unsigned long long &next, unsigned long long &don,
-                    long long *prof) {
+                    long long *prof, long long t_wave) {
   const int k = pl->k;
   int level = floor;
   long long t0 = wall_clock64();
@@ -793,6 +794,10 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       if (level == floor) return;
       level--;
       continue;
+    }
+    if (P.limit_ticks > 0 && wall_clock64() - t_wave > P.limit_ticks) {
+      if (lane_id() == 0) atomicOr(&P.stats[pid * 4 + 3], 1ull);  // this plan's count is incomplete
+      return;
     }
     if (may_donate && level < P.donate_levels) {
       const long long now = wall_clock64();
@@ -866,6 +871,7 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
   bool is_idle = false;
   long long prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const long long t_start = STATS ? clock64() : 0;
+  const long long t_wave = wall_clock64();
 
   for (;;) {
     // ---- acquire work: ring first, then the edge cursor, then wait ----
@@ -940,7 +946,10 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
       stat_pid = pid;
     }
     unsigned long long cnt = 0;
-    if (!pl->unsat) {
+    const bool expired = P.limit_ticks > 0 && wall_clock64() - t_wave > P.limit_ticks;
+    if (expired) {
+      if (lane == 0) atomicOr(&P.stats[pid * 4 + 3], 1ull);  // dropped: count incomplete
+    } else if (!pl->unsat) {
       if (kind == 1) {
         const long long tp = STATS ? clock64() : 0;
         const int n0 = filter_edges<STATS>(P, pl, e0, e1, L0r, L0c, balg);
@@ -950,7 +959,7 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           S.len[0] = n0;
           S.end[0] = n0;
           S.it[0] = 0;
-          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, cnt, balg, next, don, prof);
+          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, cnt, balg, next, don, prof, t_wave);
         }
       } else {
         const int p = t.p;
@@ -975,7 +984,7 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
           S.len[p - 1] = n;
           S.it[p - 1] = t.a;
           S.end[p - 1] = min(t.b, n);
-          dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, cnt, balg, next, don, prof);
+          dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, cnt, balg, next, don, prof, t_wave);
         }
       }
     }
@@ -1633,6 +1642,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.donate_levels = t.donate_levels;
   P.reuse = t.reuse == 1 ? 1 : 0;
   P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
+  P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   return TDFS_OK;
 }
 
@@ -1670,6 +1680,7 @@ static int launch_collect(const Launch &L, const tdfs_plan *plans, int nplans, d
     r.alg_bytes = L.h_stats[4 * i + 0];
     r.extensions = L.h_stats[4 * i + 1];
     r.donated = L.h_stats[4 * i + 2];
+    r.flags = (uint32_t)(L.h_stats[4 * i + 3] & 1ull);
     r.kernel_ms = ms;
     if (plans[i].labeled) {
       r.count = r.raw;
@@ -1838,7 +1849,7 @@ extern "C" int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slo
   sv->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 1;
   sv->max_batch = max_batch;
   sv->budget = budget;
-  sv->est_samples = 1000;
+  sv->est_samples = 4000;
   sv->slots.resize(slots);
   for (auto &sl : sv->slots) {
     int rc = launch_init(sl.L, ctx->device);

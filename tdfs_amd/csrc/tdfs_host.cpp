@@ -362,7 +362,8 @@ struct Alias {
 
 // ---------------------------------------------------------------------------
 // Work estimate: the DFS of the counting path (same level-0 filter, lists,
-// conditions and reuse) expanded from nsample uniformly drawn directed edges,
+// conditions and reuse) expanded from nsample directed edges drawn uniformly
+// within 16 equal edge-index strata (stratified estimator),
 // charging each extension the probe elements it tests times the lists it
 // tests them against.  Once the scaled estimate passes `cap` the expansion
 // stops and the estimate is marked capped (a lower bound: the pattern is then
@@ -377,6 +378,7 @@ struct Estimator {
   int32_t M[TDFS_MAX_K];
   std::vector<int32_t> cand[TDFS_MAX_K];
   double cost = 0, raw = 0, cap = 0;
+  double base = 0, scale = 1;  // scaled estimate so far = base + cost * scale
   bool capped = false;
 
   bool pass(int u, int32_t x) const {
@@ -432,7 +434,7 @@ struct Estimator {
       raw += (double)out.size();
       return;
     }
-    if (cost > cap) {
+    if (base + cost * scale > cap) {
       capped = true;
       return;
     }
@@ -470,10 +472,20 @@ int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colid
   E.colidx = colidx;
   E.labels = labels;
   E.p = plan;
-  const double scale = (double)nnz / (double)nsample;
-  E.cap = cap / scale;  // stop once the scaled estimate passes cap
-  for (int32_t s = 0; s < nsample && !E.capped; s++) {
-    const int64_t e = (int64_t)r.below((uint64_t)nnz);
+  E.cap = cap;
+  // Stratified by edge index: rows sorted by degree put the hub rows in the
+  // last strata, so every estimate samples them.
+  const int32_t nstrata = std::max<int32_t>(1, std::min<int32_t>(16, nsample));
+  const int32_t per = std::max<int32_t>(1, nsample / nstrata);
+  double raw_scaled = 0;
+  for (int32_t st = 0; st < nstrata && !E.capped; st++) {
+    const int64_t lo = nnz * st / nstrata, hi = nnz * (st + 1) / nstrata;
+    if (hi <= lo) continue;
+    E.scale = (double)(hi - lo) / (double)per;
+    E.cost = 0;
+    E.raw = 0;
+  for (int32_t s = 0; s < per && !E.capped; s++) {
+    const int64_t e = lo + (int64_t)r.below((uint64_t)(hi - lo));
     const int32_t c = colidx[e];
     const int32_t rr = (int32_t)(std::upper_bound(rowptr, rowptr + n + 1, e) - rowptr - 1);
     E.cost += 2;
@@ -487,10 +499,13 @@ int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colid
     E.M[0] = rr;
     E.M[1] = c;
     E.dfs(2);
-    if (E.cost > E.cap) E.capped = true;
+    if (E.base + E.cost * E.scale > E.cap) E.capped = true;
   }
-  out3[0] = E.cost * scale;
-  out3[1] = E.raw * scale;
+    E.base += E.cost * E.scale;
+    raw_scaled += E.raw * E.scale;
+  }
+  out3[0] = E.base;
+  out3[1] = raw_scaled;
   out3[2] = E.capped ? 1.0 : 0.0;
   return TDFS_OK;
 }

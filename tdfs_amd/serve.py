@@ -64,7 +64,7 @@ class Server:
 
 
 def query_stream(dg, count: int, seed: int, labeled_frac: float, nlabels: int, budget: float,
-                 nsample: int = 1000, max_attempts: int = 50, progress: bool = False) -> list[dict]:
+                 nsample: int = 4000, max_attempts: int = 50, progress: bool = False) -> list[dict]:
     """Seeded queries: k uniform in 3..7, edge count uniform in [k-1, k(k-1)/2],
     labeled with probability labeled_frac (nlabels labels); a draw whose
     sampled work estimate is capped or above budget is redrawn: after every
@@ -125,7 +125,9 @@ def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progre
     wall = time.perf_counter() - t0
     lats = np.array([lat[i] for i in range(len(plans))])
     counts = [int(results[i].count) for i in range(len(plans))]
+    incomplete = [i for i in range(len(plans)) if results[i].flags & 1]
     return {
+        "incomplete": incomplete,
         "rate_qps": rate,
         "queries": len(plans),
         "wall_s": wall,
@@ -151,6 +153,8 @@ def main() -> int:
     ap.add_argument("--budget", type=float, default=2e10)
     ap.add_argument("--check", type=int, default=20, help="queries recounted alone for equality")
     ap.add_argument("--queries-file", default="", help="committed query list (json) instead of drawing one")
+    ap.add_argument("--time-limit-ms", type=int, default=20000,
+                    help="per-launch limit: work still pending is dropped and its queries flagged incomplete")
     args = ap.parse_args()
     t = time.time()
     g = W.graph("c5")
@@ -168,6 +172,7 @@ def main() -> int:
     out = {"workload": "c5: Chung-Lu n=4M m=34M gamma=2.3 seed 5, 8 labels; seeded query stream (k 3-7, 30% labeled)",
            "budget": args.budget, "slots": args.slots, "blocks_per_cu": args.blocks_per_cu,
            "max_batch": args.max_batch, "streams": []}
+    dev.set_tuning(time_limit_ms=args.time_limit_ms)
     server = Server(dev, dg, args.slots, args.blocks_per_cu, args.max_batch, args.budget)
     alone = {}
     for i in range(min(args.check, len(plans))):
@@ -177,7 +182,7 @@ def main() -> int:
               file=sys.stderr, flush=True)
     for rate in [float(x) for x in args.rates.split(",")]:
         r = run_stream(server, plans, rate, 50 + int(rate), progress=25)
-        same = all(r["counts"][i] == c for i, c in alone.items())
+        same = all(r["counts"][i] == c for i, c in alone.items() if i not in r["incomplete"])
         r["alone_equal"] = same
         del r["counts"]
         out["streams"].append(r)

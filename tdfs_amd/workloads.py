@@ -11,8 +11,9 @@ c4  c3's graph; 64 connected patterns from seed 4 (k = 3 + i mod 5, edge
 c5  Chung-Lu n=4M, m=34M, gamma=2.3, seed 5, labels uniform in {0..7};
     a seeded query stream (k in 3..7, 30% labeled) under the same rule.
 
-Feasibility rule (c4, c5): the estimate of tdfs_estimate_work (2,000 edges
-sampled with seed 1, on the numbering the device counts on) must not be
+Feasibility rule (c4, c5): the estimate of tdfs_estimate_work (4,000 edges
+sampled with seed 1 in 16 edge-index strata, on the numbering the device
+counts on) must not be
 capped and must stay under `budget` list elements (c4: 1e10); a rejected
 draw is replaced by the next attempt (attempt counter in the seed) with the
 same vertex count, and after 50 rejected draws with one vertex fewer.  On
@@ -95,7 +96,7 @@ OUT_OF_REACH = {"c2": {"5-cycle": (C5, False)}, "c3": {"6-path": (P6, True), "6-
 
 
 def random_patterns(g_device: CSRGraph, count: int, seed: int, labeled_every: int, nlabels: int,
-                    budget: float, nsample: int = 2000, max_attempts: int = 50) -> list[dict]:
+                    budget: float, nsample: int = 4000, max_attempts: int = 50) -> list[dict]:
     """Seeded connected patterns kept by the feasibility rule: pattern i has
     k = 3 + i mod 5 vertices; its edge count (uniform in [k-1, k(k-1)/2]),
     edges and labels are redrawn (attempt counter in the seed) until the

@@ -270,6 +270,27 @@ def test_serving_refuses_over_budget(dev):
     sv.close()
 
 
+def test_time_limit_flags_incomplete(dev):
+    """A launch that hits its time limit drops the pending work and flags the
+    plans it belonged to; a plan that finished before stays exact."""
+    G = T.CSRGraph.chung_lu(200000, 1500000, 2.1, 8000, 71)
+    dg = dev.upload(G)
+    heavy = T.plan_pattern("v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 4 1\n", 0)
+    tri = T.plan_pattern("v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n", 0)
+    want = dev.count(dg, tri).count
+    base = dev.tuning()
+    try:
+        dev.set_tuning(time_limit_ms=200)
+        say("time limit heavy")
+        r = dev.count(dg, heavy)
+        assert r.flags & 1
+        assert r.kernel_ms < 5000
+        r = dev.count(dg, tri)
+        assert not (r.flags & 1) and r.count == want
+    finally:
+        dev.set_tuning(time_limit_ms=base.time_limit_ms)
+
+
 def test_edge_ranges_partition(dev):
     """Level-0 shards (directed-edge ranges) add up to the whole count."""
     G = T.CSRGraph.chung_lu(50000, 250000, 2.1, 4000, 41)

@@ -220,7 +220,7 @@ def test_c4_pattern_set_is_reproducible():
         assert p["est_cost"] <= 1e10
     G = W.graph("c4")
     D = G.relabeled(W.degree_order(G))
-    again = W.random_patterns(D, 6, 4, 2, 4, 1e10)
+    again = W.random_patterns(D, 6, 4, 2, 4, 1e10)  # same rule, first six indices
     assert [q["text"] for q in again] == [p["text"] for p in pats[:6]]
 
 
