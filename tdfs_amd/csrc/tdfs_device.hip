@@ -109,8 +109,8 @@ struct KParams {
   int nplans;
   int chunk;
   long long edge_lo, edge_hi;
-  unsigned long long nchunks;  // chunks per plan
-  unsigned long long total;    // nplans * nchunks * chunk
+  unsigned long long span;      // edges per plan: edge_hi - edge_lo
+  unsigned long long *cursors;  // per plan (128-byte stride): next level-0 offset
   // per-wave workspace
   int *wbuf;
   long long wstride;
@@ -872,9 +872,11 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
   long long prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const long long t_start = STATS ? clock64() : 0;
   const long long t_wave = wall_clock64();
+  int cur_plan = (int)(gw % P.nplans);
+  int plans_left = P.nplans;
 
   for (;;) {
-    // ---- acquire work: ring first, then the edge cursor, then wait ----
+    // ---- acquire work: ring first, then the edge cursors, then wait ----
     TaskRec t;
     int kind = 0;
     long long e0 = 0, e1 = 0;
@@ -883,14 +885,19 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
     if (ring_take(P, t)) {
       kind = 2;
     } else {
-      unsigned long long pos = 0;
-      if (lane == 0) pos = fadd(&C->cursor, (unsigned long long)P.chunk);
-      pos = uni64(pos);
-      if (pos < P.total) {
-        const unsigned long long c = pos / (unsigned long long)P.chunk;
-        pid = (int)(c % (unsigned long long)P.nplans);
-        const unsigned long long ci = c / (unsigned long long)P.nplans;
-        e0 = P.edge_lo + (long long)ci * P.chunk;
+      // per-plan cursors: this wave stays on its plan until that plan's
+      // edges are handed out, then moves to the next plan with edges left
+      unsigned long long pos = ~0ull;
+      while (plans_left > 0) {
+        if (lane == 0) pos = fadd(&P.cursors[(long long)cur_plan * 16], (unsigned long long)P.chunk);
+        pos = uni64(pos);
+        if (pos < P.span) break;
+        cur_plan = cur_plan + 1 == P.nplans ? 0 : cur_plan + 1;
+        plans_left--;
+      }
+      if (plans_left > 0) {
+        pid = cur_plan;
+        e0 = P.edge_lo + (long long)pos;
         e1 = min(e0 + (long long)P.chunk, P.edge_hi);
         kind = 1;
       } else {
@@ -1026,7 +1033,8 @@ __global__ void tdfs_hub_build_kernel(const long long *rowptr, const int *colidx
 // Resets the control block, results and ring sequence numbers (a kernel, so
 // that captured launch sequences hold no memset nodes).
 __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigned long long *stats, int nplans,
-                                 unsigned long long *seq, unsigned long long cap, unsigned long long *prof) {
+                                 unsigned long long *seq, unsigned long long cap, unsigned long long *prof,
+                                 unsigned long long *cursors) {
   const unsigned long long tid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned long long nth = (unsigned long long)gridDim.x * blockDim.x;
   int *cw = reinterpret_cast<int *>(ctrl);
@@ -1035,6 +1043,7 @@ __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigne
   for (unsigned long long i = tid; i < (unsigned long long)nplans * 4; i += nth) stats[i] = 0;
   for (unsigned long long i = tid; i < cap; i += nth) seq[i] = i;
   for (unsigned long long i = tid; i < 8; i += nth) prof[i] = 0;
+  for (unsigned long long i = tid; i < (unsigned long long)nplans * 16; i += nth) cursors[i] = 0;
 }
 
 }  // namespace
@@ -1092,6 +1101,7 @@ struct Launch {
   unsigned long long q_cap = 0;
   Ctrl *ctrl = nullptr;
   unsigned long long *counts = nullptr, *stats = nullptr, *prof = nullptr;
+  unsigned long long *cursors = nullptr;  // 16 words per plan
   DevPlan *plans = nullptr;
   int plan_cap = 0;
   // pinned host mirrors, written by the launch's D2H copies
@@ -1180,6 +1190,7 @@ static void launch_free(Launch &L) {
   if (L.stats) (void)hipFree(L.stats);
   if (L.prof) (void)hipFree(L.prof);
   if (L.plans) (void)hipFree(L.plans);
+  if (L.cursors) (void)hipFree(L.cursors);
   if (L.h_counts) (void)hipHostFree(L.h_counts);
   if (L.h_stats) (void)hipHostFree(L.h_stats);
   if (L.h_prof) (void)hipHostFree(L.h_prof);
@@ -1549,16 +1560,19 @@ static int launch_reserve(Launch &L, int nplans, long long waves, long long wstr
   }
   if (nplans > L.plan_cap) {
     if (L.plans) HIP_TRY(hipFree(L.plans));
+    if (L.cursors) HIP_TRY(hipFree(L.cursors));
     if (L.counts) HIP_TRY(hipFree(L.counts));
     if (L.stats) HIP_TRY(hipFree(L.stats));
     if (L.h_counts) HIP_TRY(hipHostFree(L.h_counts));
     if (L.h_stats) HIP_TRY(hipHostFree(L.h_stats));
     if (L.h_plans) HIP_TRY(hipHostFree(L.h_plans));
     L.plans = nullptr;
+    L.cursors = nullptr;
     L.counts = L.stats = L.h_counts = L.h_stats = nullptr;
     L.h_plans = nullptr;
     L.plan_cap = 0;
     HIP_TRY(hipMalloc(&L.plans, sizeof(DevPlan) * nplans));
+    HIP_TRY(hipMalloc(&L.cursors, sizeof(unsigned long long) * 16 * nplans));
     HIP_TRY(hipMalloc(&L.counts, sizeof(unsigned long long) * nplans));
     HIP_TRY(hipMalloc(&L.stats, sizeof(unsigned long long) * 4 * nplans));
     HIP_TRY(hipHostMalloc(&L.h_counts, sizeof(unsigned long long) * nplans, hipHostMallocDefault));
@@ -1588,7 +1602,6 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   if (edge_begin > edge_end) edge_begin = edge_end;
   const int chunk = t.chunk_edges;
   const long long span = edge_end - edge_begin;
-  const unsigned long long nchunks = (unsigned long long)((span + chunk - 1) / chunk);
   const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
   const long long wstride = 2ll * kChunkMax + (long long)(kMaxK - 3) * dstride;
   const int occ = stats ? ctx->blocks_per_cu_stats : ctx->blocks_per_cu;
@@ -1623,8 +1636,8 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.chunk = chunk;
   P.edge_lo = edge_begin;
   P.edge_hi = edge_end;
-  P.nchunks = nchunks;
-  P.total = (unsigned long long)nplans * nchunks * (unsigned long long)chunk;
+  P.span = (unsigned long long)span;
+  P.cursors = L.cursors;
   P.wbuf = L.wbuf;
   P.wstride = wstride;
   P.dstride = dstride;
@@ -1651,7 +1664,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
 static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
   const int init_blocks = (int)std::min<unsigned long long>(1024, (L.q_cap + 255) / 256);
   hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, L.stream, L.ctrl, L.counts, L.stats,
-                     P.nplans, L.q_seq, L.q_cap, L.prof);
+                     P.nplans, L.q_seq, L.q_cap, L.prof, L.cursors);
   HIP_TRY(hipGetLastError());
   if (stats)
     hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);

@@ -153,7 +153,7 @@ def main() -> int:
     ap.add_argument("--budget", type=float, default=2e10)
     ap.add_argument("--check", type=int, default=20, help="queries recounted alone for equality")
     ap.add_argument("--queries-file", default="", help="committed query list (json) instead of drawing one")
-    ap.add_argument("--time-limit-ms", type=int, default=20000,
+    ap.add_argument("--time-limit-ms", type=int, default=5000,
                     help="per-launch limit: work still pending is dropped and its queries flagged incomplete")
     args = ap.parse_args()
     t = time.time()
