@@ -90,7 +90,9 @@ typedef struct tdfs_tuning {
                                 the result cannot depend on the numbering (default); 2: off.  Read at upload */
   int32_t time_limit_ms;     /* > 0: work still pending this long into a launch is dropped and the plans it
                                 belonged to are flagged TDFS_INCOMPLETE (default 0: no limit) */
-  int32_t reserved[2];
+  int32_t cursor_order;      /* level-0 order of a batch: 1 one cursor over (edge chunk, plan), all plans
+                                sweep the graph together (default); 2 one cursor per plan (serving) */
+  int32_t reserved[1];
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

@@ -50,7 +50,8 @@ class Tuning(C.Structure):
         ("hub_budget_mb", C.c_int32),
         ("degree_order", C.c_int32),
         ("time_limit_ms", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("cursor_order", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 

@@ -111,6 +111,7 @@ struct KParams {
   long long edge_lo, edge_hi;
   unsigned long long span;      // edges per plan: edge_hi - edge_lo
   unsigned long long *cursors;  // per plan (128-byte stride): next level-0 offset
+  int interleave;               // 1: one cursor over (chunk, plan); 0: per-plan cursors
   // per-wave workspace
   int *wbuf;
   long long wstride;
@@ -885,10 +886,24 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
     if (ring_take(P, t)) {
       kind = 2;
     } else {
+      unsigned long long pos = ~0ull;
+      if (P.interleave) {
+        // one cursor over (edge chunk, plan) pairs: all plans sweep the same
+        // region of the graph together (shared L2 / MALL residency)
+        if (lane == 0) pos = fadd(&P.cursors[0], (unsigned long long)P.chunk);
+        pos = uni64(pos);
+        const unsigned long long c = pos / (unsigned long long)P.chunk;
+        const unsigned long long ci = c / (unsigned long long)P.nplans;
+        if (ci * (unsigned long long)P.chunk < P.span) {
+          cur_plan = (int)(c % (unsigned long long)P.nplans);
+          pos = ci * (unsigned long long)P.chunk;
+        } else {
+          plans_left = 0;
+        }
+      }
       // per-plan cursors: this wave stays on its plan until that plan's
       // edges are handed out, then moves to the next plan with edges left
-      unsigned long long pos = ~0ull;
-      while (plans_left > 0) {
+      while (!P.interleave && plans_left > 0) {
         if (lane == 0) pos = fadd(&P.cursors[(long long)cur_plan * 16], (unsigned long long)P.chunk);
         pos = uni64(pos);
         if (pos < P.span) break;
@@ -1145,6 +1160,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->hub_min_degree = 256;
   t->hub_budget_mb = 4096;
   t->degree_order = 1;
+  t->cursor_order = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1164,6 +1180,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->hub_min_degree <= 0) t->hub_min_degree = d.hub_min_degree;
   if (t->hub_budget_mb == 0) t->hub_budget_mb = d.hub_budget_mb;
   if (t->degree_order <= 0) t->degree_order = d.degree_order;
+  if (t->cursor_order <= 0) t->cursor_order = d.cursor_order;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -1638,6 +1655,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.edge_hi = edge_end;
   P.span = (unsigned long long)span;
   P.cursors = L.cursors;
+  P.interleave = t.cursor_order == 1 ? 1 : 0;
   P.wbuf = L.wbuf;
   P.wstride = wstride;
   P.dstride = dstride;
@@ -1937,6 +1955,7 @@ extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *
     }
     KParams P;
     int rc = launch_prepare(sv->ctx, sl.L, sv->g, plans.data(), (int)plans.size(), -1, -1, false, sv->blocks_per_cu, P);
+    P.interleave = 0;  // a light query finishes its own sweep without waiting for heavy batch-mates
     if (!rc) {
       HIP_TRY(hipEventRecord(sl.L.ev0, sl.L.stream));
       rc = launch_enqueue(sl.L, P, false);

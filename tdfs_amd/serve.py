@@ -145,15 +145,15 @@ def main() -> int:
     from . import workloads as W
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--rates", default="2,5,50,200,1000,2000")
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--blocks-per-cu", type=int, default=1)
-    ap.add_argument("--max-batch", type=int, default=32)
+    ap.add_argument("--max-batch", type=int, default=8)
     ap.add_argument("--budget", type=float, default=2e10)
     ap.add_argument("--check", type=int, default=20, help="queries recounted alone for equality")
     ap.add_argument("--queries-file", default="", help="committed query list (json) instead of drawing one")
-    ap.add_argument("--time-limit-ms", type=int, default=5000,
+    ap.add_argument("--time-limit-ms", type=int, default=60000,
                     help="per-launch limit: work still pending is dropped and its queries flagged incomplete")
     args = ap.parse_args()
     t = time.time()

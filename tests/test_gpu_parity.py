@@ -132,6 +132,8 @@ def test_scheduling_modes_are_exact(dev):
         dict(reuse=2),
         dict(leaf_batch=2),
         dict(leaf_batch=2, timeout_us=1, donate_levels=5),
+        dict(cursor_order=2),
+        dict(cursor_order=2, timeout_us=1, chunk_edges=16),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
@@ -194,11 +196,14 @@ def test_batch_equals_alone(dev):
     dev.set_stats(True)
     say("batch of", len(plans))
     batch = dev.count_batch(dg, plans)
+    dev.set_tuning(cursor_order=2)
+    batch2 = dev.count_batch(dg, plans)
+    dev.set_tuning(cursor_order=1)
     for j, p in enumerate(plans):
         say("alone", j)
         alone = dev.count(dg, p)
-        assert alone.raw == batch[j].raw, j
-        assert alone.alg_bytes == batch[j].alg_bytes, j
+        assert alone.raw == batch[j].raw == batch2[j].raw, j
+        assert alone.alg_bytes == batch[j].alg_bytes == batch2[j].alg_bytes, j
 
 
 def test_graph_replays_match_eager(dev):
