@@ -156,10 +156,9 @@ def main() -> int:
 
     import tdfs_amd as T
 
-    tuned = {}
     tfile = ROOT / "profiles" / "autotune.json"
-    if tfile.exists():
-        tuned = json.loads(tfile.read_text()).get(args.workload, {}).get("best", {})
+    tuned_all = json.loads(tfile.read_text()) if tfile.exists() else {}
+    tuned = tuned_all.get(args.workload, {}).get("best", {})
 
     if args.copy_calibration > 0:
         import torch
@@ -189,13 +188,16 @@ def main() -> int:
     extra = {}
     if args.workload != "c2" and not args.no_c2:
         g2, p2, d2 = load_workload("c2")
+        dev.set_tuning(**{k: 0 for k in ("waves_per_simd", "blocks_per_cu")})
+        dev.set_tuning(**tuned_all.get("c2", {}).get("best", {}))
         dg2 = dev.upload(g2)
         m2 = measure(dev, dg2, p2, args.steps, args.warmup)
         dg2.free()
         extra = {"c2": {"workload": d2, "counts": m2["counts"], "batched_matches_per_s": m2["total"] / m2["t_batched"],
                         "plain_matches_per_s": m2["total"] / m2["t_plain"], "ms_per_step": m2["t_batched"] * 1e3,
                         "kernel_ms_batched": m2["kernel_ms_batched"],
-                        "achieved_alg_gbs": m2["achieved_gbs"]}}
+                        "achieved_alg_gbs": m2["achieved_gbs"],
+                        "tuning": tuned_all.get("c2", {}).get("best", "defaults")}}
 
     cpu = None
     if not args.no_cpu:

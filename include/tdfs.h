@@ -92,7 +92,8 @@ typedef struct tdfs_tuning {
                                 belonged to are flagged TDFS_INCOMPLETE (default 0: no limit) */
   int32_t cursor_order;      /* level-0 order of a batch: 1 one cursor over (edge chunk, plan), all plans
                                 sweep the graph together (default); 2 one cursor per plan (serving) */
-  int32_t reserved[1];
+  int32_t waves_per_simd;    /* register budget of the timed kernel: 0 as compiled (default, no spills),
+                                4 or 8 waves per SIMD (spills) */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

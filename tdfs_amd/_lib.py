@@ -51,7 +51,7 @@ class Tuning(C.Structure):
         ("degree_order", C.c_int32),
         ("time_limit_ms", C.c_int32),
         ("cursor_order", C.c_int32),
-        ("reserved", C.c_int32 * 1),
+        ("waves_per_simd", C.c_int32),
     ]
 
 

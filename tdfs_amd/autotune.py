@@ -1,6 +1,7 @@
 """Autotuner for the batched launch.
 
-Coordinate descent over the launch knobs of tdfs_tuning -- persistent-grid
+Coordinate descent over the launch knobs of tdfs_tuning -- register budget
+(waves_per_simd: as compiled without spills, or 4 / 8 with), persistent-grid
 occupancy (blocks_per_cu: waves resident per CU, i.e. how many DFS stacks
 share a CU's registers and LDS), level-0 chunk (chunk_edges: edges per cursor
 grab), donation threshold (timeout_us) and donated piece size (split_chunk) --
@@ -20,6 +21,7 @@ import time
 from pathlib import Path
 
 SPACE = {
+    "waves_per_simd": [0, 4, 8],
     "blocks_per_cu": [2, 3, 4],
     "chunk_edges": [16, 32, 64, 128],
     "timeout_us": [25, 100, 200, 1000],
@@ -46,7 +48,7 @@ def time_point(dev, dg, plans, counts, knobs: dict, reps: int) -> float:
 def tune(dev, dg, plans, reps: int = 3, rounds: int = 2, log=print) -> dict:
     dev.set_stats(False)
     counts = [int(r.count) for r in dev.count_batch(dg, plans)]
-    best = {"blocks_per_cu": 0, "chunk_edges": 64, "timeout_us": 200, "split_chunk": 1}
+    best = {"waves_per_simd": 0, "blocks_per_cu": 0, "chunk_edges": 64, "timeout_us": 200, "split_chunk": 1}
     best_ms = time_point(dev, dg, plans, counts, best, reps)
     trials = [{"knobs": dict(best), "ms": best_ms}]
     log(f"start {best} {best_ms:.2f} ms")
@@ -71,7 +73,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c4")
     ap.add_argument("--out", default="profiles/autotune.json")
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--rounds", type=int, default=1)
     args = ap.parse_args()
     root = Path(__file__).resolve().parent.parent
     sys.path.insert(0, str(root))
@@ -83,7 +86,7 @@ def main() -> int:
     dg = dev.upload(g)
     plans = [T.plan_pattern(t, lab) for t, lab in pats.values()]
     t0 = time.time()
-    res = tune(dev, dg, plans, reps=args.reps, log=lambda s: print(s, flush=True))
+    res = tune(dev, dg, plans, reps=args.reps, rounds=args.rounds, log=lambda s: print(s, flush=True))
     res["workload"] = desc
     res["seconds"] = round(time.time() - t0, 1)
     out = Path(args.out)

@@ -133,6 +133,7 @@ struct KParams {
   int reuse;
   int leaf_batch;
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
+  int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
 };
 
 struct WaveLds {
@@ -855,8 +856,9 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   }
 }
 
+// The persistent match loop of one wave (body of the match kernels below).
 template <bool STATS>
-__global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
+__device__ __forceinline__ void match_body(const KParams &P) {
   __shared__ WaveLds lds[kWavesPerBlock];
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
@@ -1032,6 +1034,18 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
   }
 }
 
+// Timed / statistics kernel: registers as the compiler allocates them
+// (4 waves per SIMD, no spills).
+template <bool STATS>
+__global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
+  match_body<STATS>(P);
+}
+
+// The same loop compiled for 4 or 8 waves per SIMD: more resident DFS stacks
+// to hide memory latency, at the cost of register spills.
+__global__ void __launch_bounds__(kBlock, 4) tdfs_match_kernel_w4(KParams P) { match_body<false>(P); }
+__global__ void __launch_bounds__(kBlock, 8) tdfs_match_kernel_w8(KParams P) { match_body<false>(P); }
+
 // Sets bit y of row h for every neighbour y of hub vertex hubs[h].
 __global__ void tdfs_hub_build_kernel(const long long *rowptr, const int *colidx, const int *hubs, int nhubs,
                                       unsigned *bits, long long words) {
@@ -1134,6 +1148,8 @@ struct tdfs_ctx {
   int wallclock_khz;
   int blocks_per_cu;        // occupancy limit of the timed kernel
   int blocks_per_cu_stats;  // and of the statistics kernel
+  int blocks_per_cu_w4;     // and of the timed kernel compiled for 4 waves per SIMD
+  int blocks_per_cu_w8;     // and for 8
   tdfs_tuning tune;
   int stats;
   Launch main;              // used by tdfs_count / tdfs_count_batch
@@ -1161,6 +1177,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->hub_budget_mb = 4096;
   t->degree_order = 1;
   t->cursor_order = 1;
+  t->waves_per_simd = 0;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1181,6 +1198,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->hub_budget_mb == 0) t->hub_budget_mb = d.hub_budget_mb;
   if (t->degree_order <= 0) t->degree_order = d.degree_order;
   if (t->cursor_order <= 0) t->cursor_order = d.cursor_order;
+  if (t->waves_per_simd != 4 && t->waves_per_simd != 8) t->waves_per_simd = 0;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -1236,11 +1254,15 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
   c->wallclock_khz = khz > 0 ? khz : 100000;
-  int nb = 0, nb2 = 0;
+  int nb = 0, nb2 = 0, nb4 = 0, nb8 = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb4, tdfs_match_kernel_w4, kBlock, 0));
+  c->blocks_per_cu_w4 = std::max(1, nb4);
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdfs_match_kernel<false>, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, tdfs_match_kernel<true>, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb8, tdfs_match_kernel_w8, kBlock, 0));
   c->blocks_per_cu = std::max(1, nb);
   c->blocks_per_cu_stats = std::max(1, nb2);
+  c->blocks_per_cu_w8 = std::max(1, nb8);
   int rc = launch_init(c->main, device);
   if (rc) {
     launch_free(c->main);
@@ -1621,7 +1643,9 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   const long long span = edge_end - edge_begin;
   const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
   const long long wstride = 2ll * kChunkMax + (long long)(kMaxK - 3) * dstride;
-  const int occ = stats ? ctx->blocks_per_cu_stats : ctx->blocks_per_cu;
+  const int wv = stats ? 0 : t.waves_per_simd;
+  const int occ = stats ? ctx->blocks_per_cu_stats
+                        : (wv == 8 ? ctx->blocks_per_cu_w8 : wv == 4 ? ctx->blocks_per_cu_w4 : ctx->blocks_per_cu);
   int bpc = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, occ) : occ;
   if (blocks_cap > 0) bpc = std::min(bpc, blocks_cap);
   const int grid = ctx->cus * bpc;
@@ -1674,6 +1698,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.reuse = t.reuse == 1 ? 1 : 0;
   P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
+  P.variant = wv;
   return TDFS_OK;
 }
 
@@ -1686,6 +1711,10 @@ static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
   HIP_TRY(hipGetLastError());
   if (stats)
     hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+  else if (P.variant == 8)
+    hipLaunchKernelGGL(tdfs_match_kernel_w8, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+  else if (P.variant == 4)
+    hipLaunchKernelGGL(tdfs_match_kernel_w4, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   else
     hipLaunchKernelGGL(tdfs_match_kernel<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   HIP_TRY(hipGetLastError());
