@@ -82,7 +82,7 @@ typedef struct tdfs_tuning {
   int32_t queue_log2;        /* task ring capacity = 2^queue_log2 (default 20) */
   int32_t donate_levels;     /* donate only from stack levels < this (default 3) */
   int32_t reuse;             /* 1: result reuse (default); 2: off (testing) */
-  int32_t lds_list_max;      /* reserved */
+  int32_t leaf_pair_cap;     /* (parent, candidate) pairs per leaf batch (default 4096, >= 64) */
   int32_t leaf_batch;        /* 1: count the leaf for 64 parents at once (default); 2: one at a time */
   int32_t hub_min_degree;    /* vertices of at least this degree get a neighbour bitmap (default 256) */
   int32_t hub_budget_mb;     /* bitmap memory per uploaded graph (default 4096; <0: none); read at upload */

@@ -44,7 +44,7 @@ class Tuning(C.Structure):
         ("queue_log2", C.c_int32),
         ("donate_levels", C.c_int32),
         ("reuse", C.c_int32),
-        ("lds_list_max", C.c_int32),
+        ("leaf_pair_cap", C.c_int32),
         ("leaf_batch", C.c_int32),
         ("hub_min_degree", C.c_int32),
         ("hub_budget_mb", C.c_int32),

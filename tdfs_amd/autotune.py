@@ -4,7 +4,8 @@ Coordinate descent over the launch knobs of tdfs_tuning -- register budget
 (waves_per_simd: as compiled without spills, or 4 / 8 with), persistent-grid
 occupancy (blocks_per_cu: waves resident per CU, i.e. how many DFS stacks
 share a CU's registers and LDS), level-0 chunk (chunk_edges: edges per cursor
-grab), donation threshold (timeout_us) and donated piece size (split_chunk) --
+grab), donation threshold (timeout_us), donated piece size (split_chunk) and
+leaf batch size (leaf_pair_cap) --
 timing the hipGraph-replayed batch (median of `reps` replays per point; every
 replay's counts must equal the statistics launch).  The winner per workload is
 written to a JSON file that bench.py reads.
@@ -26,6 +27,7 @@ SPACE = {
     "chunk_edges": [16, 32, 64, 128],
     "timeout_us": [25, 100, 200, 1000],
     "split_chunk": [1, 4],
+    "leaf_pair_cap": [1024, 4096, 16384],
 }
 
 
@@ -45,16 +47,24 @@ def time_point(dev, dg, plans, counts, knobs: dict, reps: int) -> float:
     return statistics.median(ts)
 
 
-def tune(dev, dg, plans, reps: int = 3, rounds: int = 2, log=print) -> dict:
+DEFAULT = {"waves_per_simd": 0, "blocks_per_cu": 0, "chunk_edges": 64, "timeout_us": 200, "split_chunk": 1,
+           "leaf_pair_cap": 4096}
+
+
+def tune(dev, dg, plans, reps: int = 3, rounds: int = 2, log=print, start: dict | None = None,
+         knobs: list[str] | None = None) -> dict:
+    """Coordinate descent from `start` (default: the library defaults) over
+    `knobs` (default: every knob of SPACE)."""
     dev.set_stats(False)
     counts = [int(r.count) for r in dev.count_batch(dg, plans)]
-    best = {"waves_per_simd": 0, "blocks_per_cu": 0, "chunk_edges": 64, "timeout_us": 200, "split_chunk": 1}
+    best = dict(DEFAULT, **(start or {}))
+    space = {k: v for k, v in SPACE.items() if knobs is None or k in knobs}
     best_ms = time_point(dev, dg, plans, counts, best, reps)
     trials = [{"knobs": dict(best), "ms": best_ms}]
     log(f"start {best} {best_ms:.2f} ms")
     for _ in range(rounds):
         improved = False
-        for knob, values in SPACE.items():
+        for knob, values in space.items():
             for v in values:
                 if v == best[knob]:
                     continue
@@ -75,6 +85,8 @@ def main() -> int:
     ap.add_argument("--out", default="profiles/autotune.json")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--rounds", type=int, default=1)
+    ap.add_argument("--knobs", default="", help="comma-separated subset of knobs to sweep (default all)")
+    ap.add_argument("--resume", action="store_true", help="start from the best point already in --out")
     args = ap.parse_args()
     root = Path(__file__).resolve().parent.parent
     sys.path.insert(0, str(root))
@@ -85,12 +97,17 @@ def main() -> int:
     dev = T.Device(0)
     dg = dev.upload(g)
     plans = [T.plan_pattern(t, lab) for t, lab in pats.values()]
-    t0 = time.time()
-    res = tune(dev, dg, plans, reps=args.reps, rounds=args.rounds, log=lambda s: print(s, flush=True))
-    res["workload"] = desc
-    res["seconds"] = round(time.time() - t0, 1)
     out = Path(args.out)
     data = json.loads(out.read_text()) if out.exists() else {}
+    start = data.get(args.workload, {}).get("best") if args.resume else None
+    knobs = [k for k in args.knobs.split(",") if k] or None
+    t0 = time.time()
+    res = tune(dev, dg, plans, reps=args.reps, rounds=args.rounds, log=lambda s: print(s, flush=True),
+               start=start, knobs=knobs)
+    res["workload"] = desc
+    res["seconds"] = round(time.time() - t0, 1)
+    if args.resume and args.workload in data:
+        res["trials"] = data[args.workload].get("trials", []) + res["trials"]
     data[args.workload] = res
     out.write_text(json.dumps(data, indent=1))
     print(json.dumps({"workload": args.workload, "best": res["best"], "best_ms": res["best_ms"]}))

@@ -44,7 +44,6 @@ constexpr int kChunkMax = 1024;      // level-0 edges per cursor grab, upper bou
 constexpr int kWavesPerBlock = 4;    // 256-thread workgroups
 constexpr int kBlock = kWavesPerBlock * kWave;
 constexpr long long kSpinMax = 1ll << 24;  // bound on waits for a slot handshake
-constexpr int kLeafPairCap = 4096;         // (owner, candidate) pairs per leaf batch
 constexpr int kMaxSegLabels = 64;          // label segments are built for at most this many labels
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
@@ -59,6 +58,7 @@ struct DevPlan {
   int dep[8];      // reuse source vertex, -1 if none
   int num_bn_sh[8];
   int bn_sh[8][8];
+  int same_label_mask[8];  // bit j (j < u): l(j) == l(u)
 };
 
 // A donated unit of work: prefix M(0..p-1) and the iteration range [a, b) of
@@ -134,6 +134,7 @@ struct KParams {
   int leaf_batch;
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
+  int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
 };
 
 struct WaveLds {
@@ -315,6 +316,9 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   int nev[6];
   int nemask = 0;
   const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
+  // over label segments every candidate carries l(u): only path vertices of
+  // that label can coincide with one
+  const int same = (labeled && seg >= 0) ? pl->same_label_mask[u] : 0x7f;
 #pragma unroll
   for (int j = 0; j < 6; j++) {
     nev[j] = -1;
@@ -322,7 +326,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
       const int m = S.M[j];
       if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
       else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
-      else {
+      else if ((same >> j) & 1) {
         nev[j] = m;
         nemask |= 1 << j;
       }
@@ -493,6 +497,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   int nev[6];
   int nemask = 0;
   const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
+  const int same = (labeled && seg >= 0) ? pl->same_label_mask[u] : 0x7f;
 #pragma unroll
   for (int j = 0; j < 6; j++) {
     nev[j] = -1;
@@ -500,7 +505,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
       const int m = pm[j];
       if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
       else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
-      else {
+      else if ((same >> j) & 1) {
         nev[j] = m;
         nemask |= 1 << j;
       }
@@ -542,10 +547,10 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     const int t = __shfl_up(incl, d);
     if (lane >= d) incl += t;
   }
-  // Take the longest prefix of owners whose pairs fit in kLeafPairCap (at
+  // Take the longest prefix of owners whose pairs fit in leaf_pair_cap (at
   // least one owner), so that a batch stays short enough for the donation
   // check between batches to keep the tail balanced.
-  int tk = __popcll(__ballot(act && incl <= kLeafPairCap));
+  int tk = __popcll(__ballot(act && incl <= P.leaf_pair_cap));
   if (tk == 0) tk = 1;
   taken = tk;
   const bool mine = lane < tk;
@@ -1178,6 +1183,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->degree_order = 1;
   t->cursor_order = 1;
   t->waves_per_simd = 0;
+  t->leaf_pair_cap = 4096;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1199,6 +1205,8 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->degree_order <= 0) t->degree_order = d.degree_order;
   if (t->cursor_order <= 0) t->cursor_order = d.cursor_order;
   if (t->waves_per_simd != 4 && t->waves_per_simd != 8) t->waves_per_simd = 0;
+  if (t->leaf_pair_cap <= 0) t->leaf_pair_cap = d.leaf_pair_cap;
+  if (t->leaf_pair_cap < 64) t->leaf_pair_cap = 64;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -1555,6 +1563,9 @@ static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
       if (p.cond[u][j] == TDFS_GT) d.gt_mask[u] |= 1 << j;
       if (p.cond[u][j] == TDFS_LT) d.lt_mask[u] |= 1 << j;
     }
+    d.same_label_mask[u] = 0;
+    for (int j = 0; j < u; j++)
+      if (p.label[j] == p.label[u]) d.same_label_mask[u] |= 1 << j;
     d.dep[u] = p.shared_lvl[u] >= 2 ? p.shared_lvl[u] : -1;
     d.num_bn_sh[u] = d.dep[u] >= 0 ? p.num_bn_sh[u] : 0;
     for (int i = 0; i < d.num_bn_sh[u]; i++) d.bn_sh[u][i] = p.bn_sh[u][i];
@@ -1699,6 +1710,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
+  P.leaf_pair_cap = t.leaf_pair_cap;
   return TDFS_OK;
 }
 

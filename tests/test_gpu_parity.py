@@ -135,6 +135,8 @@ def test_scheduling_modes_are_exact(dev):
         dict(cursor_order=2),
         dict(waves_per_simd=8),
         dict(waves_per_simd=4, timeout_us=1),
+        dict(leaf_pair_cap=64, timeout_us=1),
+        dict(leaf_pair_cap=100000),
         dict(cursor_order=2, timeout_us=1, chunk_edges=16),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
