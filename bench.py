@@ -188,7 +188,7 @@ def main() -> int:
     extra = {}
     if args.workload != "c2" and not args.no_c2:
         g2, p2, d2 = load_workload("c2")
-        dev.set_tuning(**{k: 0 for k in ("waves_per_simd", "blocks_per_cu")})
+        dev.set_tuning(**{k: 0 for k in tuned})  # zero = library default
         dev.set_tuning(**tuned_all.get("c2", {}).get("best", {}))
         dg2 = dev.upload(g2)
         m2 = measure(dev, dg2, p2, args.steps, args.warmup)

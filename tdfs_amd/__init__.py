@@ -258,7 +258,10 @@ class Device:
 
     def set_tuning(self, **kw) -> None:
         t = self.tuning()
+        names = {f[0] for f in type(t)._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise TypeError(f"unknown tuning field {k!r}")
             setattr(t, k, v)
         check(lib().tdfs_ctx_set_tuning(self.handle, C.byref(t)), "set tuning")
 
