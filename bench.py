@@ -84,7 +84,7 @@ def cpu_baseline(g, pats, threads: int, budget_s: float = 15.0) -> dict:
     }
 
 
-def measure(dev, dg, pats, steps: int, warmup: int) -> dict:
+def measure(dev, dg, pats, steps: int, warmup: int, plain_too: bool = True) -> dict:
     """Exact counts + algorithmic bytes (statistics launch), then timed
     batched (hipGraph replay) and plain (one launch per pattern) steps."""
     import tdfs_amd as T
@@ -115,24 +115,32 @@ def measure(dev, dg, pats, steps: int, warmup: int) -> dict:
 
     for _ in range(warmup):
         batched()
-        plain()
+        if plain_too:
+            plain()
     kb, kp = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         kb.append(batched())
     t_b = (time.perf_counter() - t0) / steps
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        kp.append(plain())
-    t_p = (time.perf_counter() - t0) / steps
+    t_p = float("nan")
+    if plain_too:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            kp.append(plain())
+        t_p = (time.perf_counter() - t0) / steps
     batch.close()
     total = sum(counts)
     kernel_ms = float(np.mean(kb))
     return {
         "names": names, "counts": dict(zip(names, counts)), "total": total, "alg_bytes": alg_bytes,
-        "t_batched": t_b, "t_plain": t_p, "kernel_ms_batched": kernel_ms, "kernel_ms_plain": float(np.mean(kp)),
+        "t_batched": t_b, "t_plain": t_p, "kernel_ms_batched": kernel_ms,
+        "kernel_ms_plain": float(np.mean(kp)) if kp else float("nan"),
         "achieved_gbs": alg_bytes / (kernel_ms * 1e-3) / 1e9, "profile": prof,
     }
+
+
+def nan_none(x: float):
+    return None if x != x else x
 
 
 def main() -> int:
@@ -143,6 +151,9 @@ def main() -> int:
     ap.add_argument("--workload", default="c4")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the one-launch-per-pattern form (profiled runs: the match kernel then runs "
+                         "only as the batched launch)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a kernel of known traffic "
@@ -177,7 +188,7 @@ def main() -> int:
     if tuned:
         dev.set_tuning(**tuned)
     dg = dev.upload(g)
-    m = measure(dev, dg, pats, args.steps, args.warmup)
+    m = measure(dev, dg, pats, args.steps, args.warmup, plain_too=not args.no_plain)
     dg.free()
 
     traffic = None
@@ -222,11 +233,11 @@ def main() -> int:
         "config": {"workload": f"{args.workload}: {desc}", "patterns": len(m["names"]),
                    "total_matches": m["total"], "graph": {"n": g.n, "nnz": g.nnz, "dmax": g.dmax},
                    "tuning": tuned or "defaults"},
-        "plain_matches_per_s": m["total"] / m["t_plain"],
-        "plain_ms_per_step": m["t_plain"] * 1e3,
-        "batched_speedup": m["t_plain"] / m["t_batched"],
+        "plain_matches_per_s": nan_none(m["total"] / m["t_plain"]),
+        "plain_ms_per_step": nan_none(m["t_plain"] * 1e3),
+        "batched_speedup": nan_none(m["t_plain"] / m["t_batched"]),
         "kernel_ms_batched": m["kernel_ms_batched"],
-        "kernel_ms_plain_sum": m["kernel_ms_plain"],
+        "kernel_ms_plain_sum": nan_none(m["kernel_ms_plain"]),
         "roofline": {"bound": "hbm", "achieved": m["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": m["alg_bytes"], "kernel": "tdfs_match_kernel"},

@@ -79,7 +79,7 @@ typedef struct tdfs_tuning {
   int32_t chunk_edges;       /* level-0 directed edges per cursor grab (default 64, max 1024) */
   int32_t timeout_us;        /* donate remaining work after this long in one task (<0: never) */
   int32_t split_chunk;       /* iterations per donated task (default 1) */
-  int32_t queue_log2;        /* task ring capacity = 2^queue_log2 (default 20) */
+  int32_t queue_log2;        /* task ring capacity = 2^queue_log2 (default 18, 8..26) */
   int32_t donate_levels;     /* donate only from stack levels < this (default 3) */
   int32_t reuse;             /* 1: result reuse (default); 2: off (testing) */
   int32_t leaf_pair_cap;     /* (parent, candidate) pairs per leaf batch (default 4096, >= 64) */
@@ -94,6 +94,9 @@ typedef struct tdfs_tuning {
                                 sweep the graph together (default); 2 one cursor per plan (serving) */
   int32_t waves_per_simd;    /* register budget of the timed kernel: 0 as compiled (default, no spills),
                                 4 or 8 waves per SIMD (spills) */
+  int32_t hoist_min;         /* the inputs of C(u) that do not involve M(u-1) are intersected once per
+                                M(u-2) when the level holds at least this many candidates (default 2;
+                                < 0: never) */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

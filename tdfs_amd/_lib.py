@@ -52,6 +52,7 @@ class Tuning(C.Structure):
         ("time_limit_ms", C.c_int32),
         ("cursor_order", C.c_int32),
         ("waves_per_simd", C.c_int32),
+        ("hoist_min", C.c_int32),
     ]
 
 

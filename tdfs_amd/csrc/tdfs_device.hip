@@ -59,7 +59,19 @@ struct DevPlan {
   int num_bn_sh[8];
   int bn_sh[8][8];
   int same_label_mask[8];  // bit j (j < u): l(j) == l(u)
+  // Hoisted partial intersection H(u) (u >= 3), index [0] with result reuse,
+  // [1] without: the inputs of C(u) that do not involve M(u-1) (C(dep) under
+  // reuse, and the adjacency lists hx[..][u][0 .. hx_n-1]), filtered by the
+  // conditions against M(0..u-2).  hoist 1: C(u) = H(u) ∩ N(M(u-1)) filtered
+  // against M(u-1); 2: C(u) = H(u) filtered against M(u-1); 0: not hoisted.
+  int hoist[2][8];
+  int hx_n[2][8];
+  int hx[2][8][8];
 };
+
+// Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
+// (kHoisted), or H(u) itself (kBuildH).
+enum { kFull = 0, kHoisted = 1, kBuildH = 2 };
 
 // A donated unit of work: prefix M(0..p-1) and the iteration range [a, b) of
 // C(p).  fresh = 1 marks level-0 donations whose first extension was never
@@ -135,6 +147,7 @@ struct KParams {
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
+  int hoist_min;             // build H(u) for a level of at least this many candidates (0: never)
 };
 
 struct WaveLds {
@@ -142,6 +155,8 @@ struct WaveLds {
   int it[8];
   int end[8];
   int len[8];
+  int hv[8];  // H(u) built for the current M(0..u-2)
+  int hl[8];  // |H(u)|
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -234,11 +249,18 @@ __device__ __forceinline__ const int *adj_list(const KParams &P, int t, int lab,
 // Candidate generation for pattern vertex u (2 <= u <= k-1) given M(0..u-1).
 // Returns |C(u)|; writes C(u) (ascending) to out unless out == nullptr.
 // ---------------------------------------------------------------------------
+//
+// mode kFull: the plan's inputs -- C(dep) and N(M(b)) for b in BN_sh(u) under
+// result reuse, else N(M(b)) for b in BN(u).  kHoisted: H(u) (and
+// N(M(u-1)) when hoist = 1).  kBuildH: writes H(u), filtering only against
+// M(0..u-2).  Statistics always count the plan's inputs (the merge model),
+// whichever lists are read.
 template <bool STATS>
 __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int u,
-                           int *__restrict__ out, const int *lbase, bool count_stats,
+                           int *__restrict__ out, const int *lbase, const int *hbase, int mode, bool count_stats,
                            unsigned long long &balg, unsigned long long &next) {
   const int lane = lane_id();
+  const int ri = P.reuse ? 0 : 1;
   const int dep = P.reuse ? pl->dep[u] : -1;
   const int *lp[7];
   const unsigned *hb[7];
@@ -254,13 +276,61 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   // labeled plans read only the label-l(u) segment of each adjacency list
   const int seg = (pl->labeled && P.colidx_lab) ? pl->label[u] : -1;
   int minfull = INT_MAX;  // statistics: shortest full input list
-  if (dep >= 0) {
+  if (STATS && count_stats) {
+    if (dep >= 0) {
+      const int l = S.len[dep - 1];
+      bytes += 4ull * (unsigned)l;
+      minfull = l;
+      const int nsh = pl->num_bn_sh[u];
+      for (int i = 0; i < nsh; i++) {
+        const int d = P.degv[S.M[pl->bn_sh[u][i]]];
+        bytes += 4ull * (unsigned)d + 16;
+        minfull = min(minfull, d);
+      }
+    } else {
+      const int nb = pl->num_bn[u];
+      for (int i = 0; i < nb; i++) {
+        const int d = P.degv[S.M[pl->bn[u][i]]];
+        bytes += 4ull * (unsigned)d + 16;
+        minfull = min(minfull, d);
+      }
+    }
+  }
+  if (mode == kHoisted) {
+    lp[0] = hbase + (long long)(u - 3) * P.dstride;
+    ln[0] = S.hl[u];
+    nl = 1;
+    if (pl->hoist[ri][u] == 1) {
+      const int t = S.M[u - 1];
+      lp[1] = adj_list(P, t, seg, ln[1]);
+      hb[1] = hub_row(P, t);
+      nl = 2;
+    }
+  } else if (mode == kBuildH) {
+    const int nx = pl->hx_n[ri][u];
+    int off = 0;
+    if (dep >= 0) {
+      lp[0] = lbase + (long long)(dep - 2) * P.dstride;
+      ln[0] = S.len[dep - 1];
+      off = 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      if (i < nx) {
+        const int t = S.M[pl->hx[ri][u][i]];
+        if (off) {
+          lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
+          hb[i + 1] = hub_row(P, t);
+        } else {
+          lp[i] = adj_list(P, t, seg, ln[i]);
+          hb[i] = hub_row(P, t);
+        }
+      }
+    }
+    nl = off + nx;
+  } else if (dep >= 0) {
     lp[0] = lbase + (long long)(dep - 2) * P.dstride;  // C(dep) is the list of level dep-1
     ln[0] = S.len[dep - 1];
-    if (STATS) {
-      bytes += 4ull * (unsigned)ln[0];
-      minfull = ln[0];
-    }
     const int nsh = pl->num_bn_sh[u];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
@@ -268,11 +338,6 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         const int t = S.M[pl->bn_sh[u][i]];
         lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
         hb[i + 1] = hub_row(P, t);
-        if (STATS) {
-          const int d = P.degv[t];
-          bytes += 4ull * (unsigned)d + 16;
-          minfull = min(minfull, d);
-        }
       }
     }
     nl = 1 + nsh;
@@ -284,11 +349,6 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         const int t = S.M[pl->bn[u][i]];
         lp[i] = adj_list(P, t, seg, ln[i]);
         hb[i] = hub_row(P, t);
-        if (STATS) {
-          const int d = P.degv[t];
-          bytes += 4ull * (unsigned)d + 16;
-          minfull = min(minfull, d);
-        }
       }
     }
     nl = nb;
@@ -305,13 +365,16 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
       pidx = i;
     }
   }
-  const bool leaf = (u == pl->k - 1);
+  const bool leaf = (u == pl->k - 1) && mode != kBuildH;
   const bool labeled = pl->labeled != 0;
   if (STATS && count_stats) {
     if (labeled) bytes += (unsigned)minfull;
   }
   const bool label_check = labeled && seg < 0;
-  // filter bounds
+  // every probe element carries l(u): segments, C(dep) (same label as u) or H(u)
+  const bool label_known = !labeled || seg >= 0 || dep >= 0 || mode == kHoisted;
+  // filter bounds; H(u) is filtered only against M(0..u-2)
+  const int jlim = mode == kBuildH ? u - 1 : u;
   int lo = -1, hi = INT_MAX;
   int nev[6];
   int nemask = 0;
@@ -322,7 +385,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
 #pragma unroll
   for (int j = 0; j < 6; j++) {
     nev[j] = -1;
-    if (j < u) {
+    if (j < jlim) {
       const int m = S.M[j];
       if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
       else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
@@ -337,7 +400,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
   unsigned n = 0;
   if (start < stop) {
-    if (leaf && nl == 1 && !labeled) {
+    if (leaf && nl == 1 && label_known) {
       // Count-only leaf over one list: |list ∩ (lo,hi)| minus the excluded
       // path vertices that lie in that window and in the list.
       n = (unsigned)(stop - start);
@@ -397,11 +460,12 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // then flattened into one pair space (wave prefix scan) and each lane tests
 // one (owner, candidate) pair per step, finding its owner by a binary search
 // over the scan with ds_bpermute.  Same result and statistics as calling
-// extend() for each element in turn.
+// extend() for each element in turn (with H(k-1) when it is built).
 template <bool STATS>
 __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int L,
                                          int i0, int i1, const int *L0r, const int *L0c, const int *lbase,
-                                         unsigned long long &balg, unsigned long long &next, int &taken) {
+                                         const int *hbase, unsigned long long &balg, unsigned long long &next,
+                                         int &taken) {
   const int lane = lane_id();
   const int k = pl->k;
   const int u = k - 1;
@@ -427,7 +491,10 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
       if (b == j) t = pm[j];
     return t;
   };
+  const int ri = P.reuse ? 0 : 1;
   const int dep = P.reuse ? pl->dep[u] : -1;
+  // H(u) serves every owner of the batch (L > 0: M(0..u-2) are shared)
+  const bool hoisted = L > 0 && S.hv[u] != 0;
   const int *lp[7];
   const unsigned *hb[7];
   int ln[7];
@@ -441,13 +508,45 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   unsigned long long bytes = 0;
   const int seg = (labeled && P.colidx_lab) ? pl->label[u] : -1;
   int minfull = INT_MAX;
-  if (dep >= 0) {
+  if (STATS) {
+    if (dep >= 0) {
+      const int l = S.len[dep - 1];
+      bytes += 4ull * (unsigned)l;
+      minfull = l;
+      const int nsh = pl->num_bn_sh[u];
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        if (i < nsh) {
+          const int d = P.degv[pick(pl->bn_sh[u][i])];
+          bytes += 4ull * (unsigned)d + 16;
+          minfull = min(minfull, d);
+        }
+      }
+    } else {
+      const int nb = pl->num_bn[u];
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        if (i < nb) {
+          const int d = P.degv[pick(pl->bn[u][i])];
+          bytes += 4ull * (unsigned)d + 16;
+          minfull = min(minfull, d);
+        }
+      }
+    }
+  }
+  if (hoisted) {
+    lp[0] = hbase + (long long)(u - 3) * P.dstride;
+    ln[0] = S.hl[u];
+    nl = 1;
+    if (pl->hoist[ri][u] == 1) {
+      const int t = pick(u - 1);
+      lp[1] = adj_list(P, t, seg, ln[1]);
+      hb[1] = hub_row(P, t);
+      nl = 2;
+    }
+  } else if (dep >= 0) {
     lp[0] = lbase + (long long)(dep - 2) * P.dstride;
     ln[0] = S.len[dep - 1];
-    if (STATS) {
-      bytes += 4ull * (unsigned)ln[0];
-      minfull = ln[0];
-    }
     const int nsh = pl->num_bn_sh[u];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
@@ -455,11 +554,6 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
         const int t = pick(pl->bn_sh[u][i]);
         lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
         hb[i + 1] = hub_row(P, t);
-        if (STATS) {
-          const int d = P.degv[t];
-          bytes += 4ull * (unsigned)d + 16;
-          minfull = min(minfull, d);
-        }
       }
     }
     nl = 1 + nsh;
@@ -471,11 +565,6 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
         const int t = pick(pl->bn[u][i]);
         lp[i] = adj_list(P, t, seg, ln[i]);
         hb[i] = hub_row(P, t);
-        if (STATS) {
-          const int d = P.degv[t];
-          bytes += 4ull * (unsigned)d + 16;
-          minfull = min(minfull, d);
-        }
       }
     }
     nl = nb;
@@ -518,7 +607,9 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
     if (stop < start) stop = start;
   }
-  if (!labeled && nl == 1) {
+  // every probe element carries l(u): segments, C(dep) (same label as u) or H(u)
+  const bool label_known = !labeled || seg >= 0 || dep >= 0 || hoisted;
+  if (nl == 1 && label_known) {
     // |list ∩ (lo,hi)| minus excluded path vertices inside the window
     taken = i1 - i0;
     if (STATS) {
@@ -782,12 +873,31 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
   return ntasks;
 }
 
+// Before iterating the n candidates of C(un-1): builds H(un) when the plan
+// hoists un and n reaches hoist_min, and records whether it did.  Returns
+// false when H(un) is empty, so that no C(un) of this level can be non-empty
+// (the timed kernel then skips the level; the statistics kernel walks it).
+template <bool STATS>
+__device__ __forceinline__ bool prepare_hoist(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int un,
+                                              int n, const int *lbase, int *hbase) {
+  int hv = 0, hn = 0;
+  if (un < pl->k && pl->hoist[P.reuse ? 0 : 1][un] && P.hoist_min > 0 && n >= P.hoist_min) {
+    unsigned long long db = 0, dn = 0;
+    hn = (int)extend<STATS>(P, pl, S, un, hbase + (long long)(un - 3) * P.dstride, lbase, hbase, kBuildH, false, db,
+                            dn);
+    hv = 1;
+  }
+  S.hv[un] = hv;
+  S.hl[un] = hn;
+  return STATS || hv == 0 || hn > 0;
+}
+
 // Iterative DFS from stack level `floor` (its list, iterator and end already
 // set).  Stack level L iterates C(L+1) (L >= 1) or the edge pairs (L = 0);
 // taking element x at level L fixes M(L+1) = x and computes C(L+2).
 template <bool STATS>
 __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
-                    const int *L0r, const int *L0c, int *lbase, unsigned long long &cnt,
+                    const int *L0r, const int *L0c, int *lbase, int *hbase, unsigned long long &cnt,
                     unsigned long long &balg, unsigned long long &next, unsigned long long &don,
                     long long *prof, long long t_wave) {
   const int k = pl->k;
@@ -830,7 +940,8 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     if (level + 2 == k - 1 && P.leaf_batch) {
       int taken = 0;
       const long long tp = STATS ? clock64() : 0;
-      cnt += leaf_batch<STATS>(P, pl, S, level, it, min(end, it + kWave), L0r, L0c, lbase, balg, next, taken);
+      cnt += leaf_batch<STATS>(P, pl, S, level, it, min(end, it + kWave), L0r, L0c, lbase, hbase, balg, next,
+                               taken);
       if (STATS) prof[2] += clock64() - tp;
       S.it[level] = it + taken;
       continue;
@@ -843,21 +954,22 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       S.M[level + 1] = lbase[(long long)(level - 1) * P.dstride + it];
     }
     const int u = level + 2;
+    const int mode = S.hv[u] ? kHoisted : kFull;
     const long long tp = STATS ? clock64() : 0;
     if (u == k - 1) {
-      cnt += extend<STATS>(P, pl, S, u, nullptr, lbase, true, balg, next);
+      cnt += extend<STATS>(P, pl, S, u, nullptr, lbase, hbase, mode, true, balg, next);
       if (STATS) prof[2] += clock64() - tp;
       continue;
     }
     int *out = lbase + (long long)level * P.dstride;  // list of level+1
-    const int n = (int)extend<STATS>(P, pl, S, u, out, lbase, true, balg, next);
-    if (STATS) prof[1] += clock64() - tp;
+    const int n = (int)extend<STATS>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
     if (n > 0) {
       S.len[level + 1] = n;
       S.end[level + 1] = n;
       S.it[level + 1] = 0;
-      level++;
+      if (prepare_hoist<STATS>(P, pl, S, u + 1, n, lbase, hbase)) level++;
     }
+    if (STATS) prof[1] += clock64() - tp;
   }
 }
 
@@ -871,8 +983,10 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   int *wb = P.wbuf + gw * P.wstride;
   int *L0r = wb;
   int *L0c = wb + kChunkMax;
-  int *lbase = wb + 2 * kChunkMax;
+  int *lbase = wb + 2 * kChunkMax;                          // C(2..k-2): stack levels 1..k-3
+  int *hbase = lbase + (long long)(kMaxK - 3) * P.dstride;  // H(3..k-1)
   WaveLds &S = lds[wib];
+  if (lane < 8) S.hv[lane] = 0;
   Ctrl *C = P.ctrl;
   unsigned long long balg = 0, next = 0, don = 0;
   int stat_pid = -1;
@@ -988,7 +1102,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           S.len[0] = n0;
           S.end[0] = n0;
           S.it[0] = 0;
-          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, cnt, balg, next, don, prof, t_wave);
+          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
         }
       } else {
         const int p = t.p;
@@ -999,21 +1113,22 @@ __device__ __forceinline__ void match_body(const KParams &P) {
         unsigned long long dummy_b = 0, dummy_n = 0;
         const long long tp = STATS ? clock64() : 0;
         for (int u = 2; u < p; u++) {
-          const int n = (int)extend<STATS>(P, pl, S, u, lbase + (long long)(u - 2) * P.dstride, lbase, false,
-                                           dummy_b, dummy_n);
+          const int n = (int)extend<STATS>(P, pl, S, u, lbase + (long long)(u - 2) * P.dstride, lbase, hbase, kFull,
+                                           false, dummy_b, dummy_n);
           S.len[u - 1] = n;
         }
         if (STATS) prof[4] += clock64() - tp;
         const bool fresh = t.fresh != 0;
         if (p == k - 1) {
-          cnt += extend<STATS>(P, pl, S, p, nullptr, lbase, fresh, balg, next);
+          cnt += extend<STATS>(P, pl, S, p, nullptr, lbase, hbase, kFull, fresh, balg, next);
         } else {
-          const int n = (int)extend<STATS>(P, pl, S, p, lbase + (long long)(p - 2) * P.dstride, lbase, fresh,
-                                           balg, next);
+          const int n = (int)extend<STATS>(P, pl, S, p, lbase + (long long)(p - 2) * P.dstride, lbase, hbase, kFull,
+                                           fresh, balg, next);
           S.len[p - 1] = n;
           S.it[p - 1] = t.a;
           S.end[p - 1] = min(t.b, n);
-          dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, cnt, balg, next, don, prof, t_wave);
+          if (n > 0 && prepare_hoist<STATS>(P, pl, S, p + 1, n, lbase, hbase))
+            dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
         }
       }
     }
@@ -1184,6 +1299,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->cursor_order = 1;
   t->waves_per_simd = 0;
   t->leaf_pair_cap = 4096;
+  t->hoist_min = 2;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1207,6 +1323,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->waves_per_simd != 4 && t->waves_per_simd != 8) t->waves_per_simd = 0;
   if (t->leaf_pair_cap <= 0) t->leaf_pair_cap = d.leaf_pair_cap;
   if (t->leaf_pair_cap < 64) t->leaf_pair_cap = 64;
+  if (t->hoist_min == 0) t->hoist_min = d.hoist_min;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -1570,6 +1687,23 @@ static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
     d.num_bn_sh[u] = d.dep[u] >= 0 ? p.num_bn_sh[u] : 0;
     for (int i = 0; i < d.num_bn_sh[u]; i++) d.bn_sh[u][i] = p.bn_sh[u][i];
   }
+  // H(u): the inputs of C(u) other than N(M(u-1)); worth a list of its own
+  // when at least two of them remain to intersect
+  for (int ri = 0; ri < 2; ri++) {
+    for (int u = 3; u < p.k; u++) {
+      const int dep = ri == 0 ? d.dep[u] : -1;
+      const int n_in = dep >= 0 ? d.num_bn_sh[u] : d.num_bn[u];
+      const int *in = dep >= 0 ? d.bn_sh[u] : d.bn[u];
+      bool has_prev = false;
+      int nx = 0;
+      for (int i = 0; i < n_in; i++) {
+        if (in[i] == u - 1) has_prev = true;
+        else d.hx[ri][u][nx++] = in[i];
+      }
+      d.hx_n[ri][u] = nx;
+      d.hoist[ri][u] = (dep >= 0 ? 1 : 0) + nx >= 2 ? (has_prev ? 1 : 2) : 0;
+    }
+  }
 }
 
 static int check_plan(const tdfs_plan &p) {
@@ -1653,7 +1787,8 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   const int chunk = t.chunk_edges;
   const long long span = edge_end - edge_begin;
   const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
-  const long long wstride = 2ll * kChunkMax + (long long)(kMaxK - 3) * dstride;
+  // level-0 pairs, C(2..5), H(3..6)
+  const long long wstride = 2ll * kChunkMax + 2ll * (kMaxK - 3) * dstride;
   const int wv = stats ? 0 : t.waves_per_simd;
   const int occ = stats ? ctx->blocks_per_cu_stats
                         : (wv == 8 ? ctx->blocks_per_cu_w8 : wv == 4 ? ctx->blocks_per_cu_w4 : ctx->blocks_per_cu);
@@ -1711,6 +1846,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
   P.leaf_pair_cap = t.leaf_pair_cap;
+  P.hoist_min = t.hoist_min > 0 ? t.hoist_min : 0;
   return TDFS_OK;
 }
 

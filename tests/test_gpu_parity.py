@@ -11,6 +11,7 @@ import pytest
 
 from oracle import oracle as O
 import tdfs_amd as T
+from tdfs_amd.workloads import with_pattern_labels
 
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -138,6 +139,11 @@ def test_scheduling_modes_are_exact(dev):
         dict(leaf_pair_cap=64, timeout_us=1),
         dict(leaf_pair_cap=100000),
         dict(cursor_order=2, timeout_us=1, chunk_edges=16),
+        dict(hoist_min=-1),
+        dict(hoist_min=1, timeout_us=1, donate_levels=5),
+        dict(hoist_min=1, reuse=2),
+        dict(hoist_min=1, leaf_batch=2),
+        dict(hoist_min=64),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
@@ -145,6 +151,30 @@ def test_scheduling_modes_are_exact(dev):
                 say("sched", f"{i}.g", labeled, mode)
                 r = check(dev, dg, G, fixture(i), labeled, stats=mode.get("reuse") != 2, **mode)
                 dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_ if f != "reserved"})
+
+
+def test_many_labels_without_segments(dev):
+    """More than 64 distinct labels: no label segments are built and every
+    candidate is label-tested; counts and statistics still match the oracle,
+    with and without hoisted intersections."""
+    base = T.CSRGraph.chung_lu(4000, 24000, 2.1, 400, 41)
+    rng = np.random.default_rng(41)
+    labels = rng.integers(0, 2, size=base.n).astype(np.int32)
+    rare = rng.choice(base.n, size=200, replace=False)
+    labels[rare] = rng.integers(2, 100, size=rare.shape[0])
+    G = T.CSRGraph(base.rowptr, base.colidx, labels)
+    dg = dev.upload(G)
+    base_t = dev.tuning()
+    for i in (1, 2, 5, 7, 11, 13):
+        text = fixture(i)
+        k = sum(1 for l in text.splitlines() if l.startswith("v"))
+        labs = np.random.default_rng([41, i]).integers(0, 2, size=k)
+        lt = with_pattern_labels(text, labs)
+        for mode in (dict(), dict(hoist_min=-1), dict(hoist_min=1, timeout_us=1, donate_levels=5)):
+            say("many labels", f"{i}.g", mode)
+            check(dev, dg, G, lt, 1, **mode)
+            dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_ if f != "reserved"})
+    dg.free()
 
 
 def test_hub_beyond_reference_buffer(dev):
