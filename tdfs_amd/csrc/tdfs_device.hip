@@ -61,10 +61,13 @@ struct DevPlan {
   int same_label_mask[8];  // bit j (j < u): l(j) == l(u)
   // Hoisted partial intersection H(u) (u >= 3), index [0] with result reuse,
   // [1] without: the inputs of C(u) that do not involve M(u-1) (C(dep) under
-  // reuse, and the adjacency lists hx[..][u][0 .. hx_n-1]), filtered by the
-  // conditions against M(0..u-2).  hoist 1: C(u) = H(u) ∩ N(M(u-1)) filtered
-  // against M(u-1); 2: C(u) = H(u) filtered against M(u-1); 0: not hoisted.
+  // reuse, and the adjacency lists hx[..][u][0 .. hx_n-1]).  They depend on
+  // M(0..hfix) only, so H(u) is built, filtered by the conditions against
+  // M(0..hfix), as soon as M(hfix) is fixed, and serves every M(hfix+1..u-1)
+  // below it.  hoist 1: C(u) = H(u) ∩ N(M(u-1)); 2: C(u) = H(u); both then
+  // filtered against all of M(0..u-1).  0: not hoisted.
   int hoist[2][8];
+  int hfix[2][8];
   int hx_n[2][8];
   int hx[2][8][8];
 };
@@ -155,7 +158,7 @@ struct WaveLds {
   int it[8];
   int end[8];
   int len[8];
-  int hv[8];  // H(u) built for the current M(0..u-2)
+  int hv[8];  // H(u) built for the current M(0..hfix(u))
   int hl[8];  // |H(u)|
 };
 
@@ -373,8 +376,8 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   const bool label_check = labeled && seg < 0;
   // every probe element carries l(u): segments, C(dep) (same label as u) or H(u)
   const bool label_known = !labeled || seg >= 0 || dep >= 0 || mode == kHoisted;
-  // filter bounds; H(u) is filtered only against M(0..u-2)
-  const int jlim = mode == kBuildH ? u - 1 : u;
+  // filter bounds; H(u) is filtered only against M(0..hfix(u))
+  const int jlim = mode == kBuildH ? pl->hfix[ri][u] + 1 : u;
   int lo = -1, hi = INT_MAX;
   int nev[6];
   int nemask = 0;
@@ -873,23 +876,37 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
   return ntasks;
 }
 
-// Before iterating the n candidates of C(un-1): builds H(un) when the plan
-// hoists un and n reaches hoist_min, and records whether it did.  Returns
-// false when H(un) is empty, so that no C(un) of this level can be non-empty
-// (the timed kernel then skips the level; the statistics kernel walks it).
+// After M(0..fhi) are fixed and the n candidates of the next level are
+// known: for every u >= umin whose H(u) depends on M(0..f), flo <= f <= fhi,
+// builds H(u) when n reaches hoist_min (else marks it unbuilt).  Returns false
+// when a built H(u) is empty: then no C(u) below this prefix is non-empty
+// (the timed kernel skips the subtree; the statistics kernel walks it).
 template <bool STATS>
-__device__ __forceinline__ bool prepare_hoist(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int un,
-                                              int n, const int *lbase, int *hbase) {
-  int hv = 0, hn = 0;
-  if (un < pl->k && pl->hoist[P.reuse ? 0 : 1][un] && P.hoist_min > 0 && n >= P.hoist_min) {
-    unsigned long long db = 0, dn = 0;
-    hn = (int)extend<STATS>(P, pl, S, un, hbase + (long long)(un - 3) * P.dstride, lbase, hbase, kBuildH, false, db,
-                            dn);
-    hv = 1;
+__device__ __forceinline__ bool prepare_hoists(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int flo,
+                                               int fhi, int umin, int n, const int *lbase, int *hbase) {
+  const int ri = P.reuse ? 0 : 1;
+  bool live = true;
+  for (int u = umin; u < pl->k; u++) {
+    const int f = pl->hfix[ri][u];
+    if (!pl->hoist[ri][u] || f < flo || f > fhi) continue;
+    int hv = 0, hn = 0;
+    if (P.hoist_min > 0 && n >= P.hoist_min) {
+      unsigned long long db = 0, dn = 0;
+      hn = (int)extend<STATS>(P, pl, S, u, hbase + (long long)(u - 3) * P.dstride, lbase, hbase, kBuildH, false, db,
+                              dn);
+      hv = 1;
+      if (hn == 0) live = false;
+    }
+    S.hv[u] = hv;
+    S.hl[u] = hn;
   }
-  S.hv[un] = hv;
-  S.hl[un] = hn;
-  return STATS || hv == 0 || hn > 0;
+  return STATS || live;
+}
+
+// Marks every H(u) unbuilt (start of a task: the plan or prefix changed).
+__device__ __forceinline__ void clear_hoists(WaveLds &S) {
+#pragma unroll
+  for (int u = 0; u < 8; u++) S.hv[u] = 0;
 }
 
 // Iterative DFS from stack level `floor` (its list, iterator and end already
@@ -967,7 +984,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       S.len[level + 1] = n;
       S.end[level + 1] = n;
       S.it[level + 1] = 0;
-      if (prepare_hoist<STATS>(P, pl, S, u + 1, n, lbase, hbase)) level++;
+      if (prepare_hoists<STATS>(P, pl, S, u - 1, u - 1, u + 1, n, lbase, hbase)) level++;
     }
     if (STATS) prof[1] += clock64() - tp;
   }
@@ -986,7 +1003,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   int *lbase = wb + 2 * kChunkMax;                          // C(2..k-2): stack levels 1..k-3
   int *hbase = lbase + (long long)(kMaxK - 3) * P.dstride;  // H(3..k-1)
   WaveLds &S = lds[wib];
-  if (lane < 8) S.hv[lane] = 0;
+  clear_hoists(S);
   Ctrl *C = P.ctrl;
   unsigned long long balg = 0, next = 0, don = 0;
   int stat_pid = -1;
@@ -1098,6 +1115,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
         const int n0 = filter_edges<STATS>(P, pl, e0, e1, L0r, L0c, balg);
         if (STATS) prof[0] += clock64() - tp;
         if (n0 > 0) {
+          clear_hoists(S);
           if (STATS) next += (unsigned)n0;
           S.len[0] = n0;
           S.end[0] = n0;
@@ -1127,7 +1145,8 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           S.len[p - 1] = n;
           S.it[p - 1] = t.a;
           S.end[p - 1] = min(t.b, n);
-          if (n > 0 && prepare_hoist<STATS>(P, pl, S, p + 1, n, lbase, hbase))
+          clear_hoists(S);
+          if (n > 0 && prepare_hoists<STATS>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase))
             dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
         }
       }
@@ -1702,6 +1721,10 @@ static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
       }
       d.hx_n[ri][u] = nx;
       d.hoist[ri][u] = (dep >= 0 ? 1 : 0) + nx >= 2 ? (has_prev ? 1 : 2) : 0;
+      // the last vertex the hoisted inputs depend on (C(dep) on M(0..dep-1))
+      int f = dep >= 0 ? dep - 1 : -1;
+      for (int i = 0; i < nx; i++) f = std::max(f, d.hx[ri][u][i]);
+      d.hfix[ri][u] = d.hoist[ri][u] ? std::max(f, 1) : -1;
     }
   }
 }

@@ -177,6 +177,26 @@ def test_many_labels_without_segments(dev):
     dg.free()
 
 
+def test_dense_labeled_patterns_hoisted(dev):
+    """Dense 5-/6-vertex patterns of the c4 set, whose hoisted intersections
+    depend on an early prefix only (built once per M(1) or M(2) and shared by
+    the levels below), counted with hoisting on, off and forced, with
+    donation; counts and statistics equal the oracle's."""
+    G = T.CSRGraph.chung_lu(3000, 24000, 2.1, 300, 71).with_labels(4, 7)
+    dg = dev.upload(G)
+    base_t = dev.tuning()
+    pats = json.loads((GOLDEN / "c4_patterns.json").read_text())["patterns"]
+    for idx in (3, 23, 29, 33, 49, 13):
+        p = pats[idx]
+        for labeled in sorted({0, int(p["labeled"])}):
+            for mode in (dict(), dict(hoist_min=-1), dict(hoist_min=1, timeout_us=1, donate_levels=5),
+                         dict(hoist_min=1, leaf_batch=2)):
+                say("dense", idx, labeled, mode)
+                check(dev, dg, G, p["text"], labeled, **mode)
+                dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_ if f != "reserved"})
+    dg.free()
+
+
 def test_hub_beyond_reference_buffer(dev):
     """A hub of degree 6000 > 4096 (the reference's per-level slot size,
     src/config.h:12): lists are never truncated here."""
