@@ -31,6 +31,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# timed match kernel per tdfs_tuning.waves_per_simd
+KERNEL_NAMES = {0: "tdfs_match_kernel", 4: "tdfs_match_kernel_w4", 8: "tdfs_match_kernel_w8"}
 
 
 def load_workload(name: str):
@@ -240,7 +242,8 @@ def main() -> int:
         "kernel_ms_plain_sum": nan_none(m["kernel_ms_plain"]),
         "roofline": {"bound": "hbm", "achieved": m["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": m["alg_bytes"], "kernel": "tdfs_match_kernel"},
+                     "alg_bytes_per_launch": m["alg_bytes"],
+                     "kernel": KERNEL_NAMES.get(int(tuned.get("waves_per_simd", 0)), "tdfs_match_kernel")},
         "phase_cycles_frac": {k: round(v / tot, 4) for k, v in prof.items() if k not in ("total", "waves")},
         "cpu_baseline": cpu,
         **extra,
