@@ -160,6 +160,10 @@ struct WaveLds {
   int len[8];
   int hv[8];  // H(u) built for the current M(0..hfix(u))
   int hl[8];  // |H(u)|
+  // wave-uniform input lists of the extension in progress
+  const int *ulp[8];
+  const unsigned *uhb[8];
+  int uln[8];
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -251,30 +255,22 @@ __device__ __forceinline__ const int *adj_list(const KParams &P, int t, int lab,
 // ---------------------------------------------------------------------------
 // Candidate generation for pattern vertex u (2 <= u <= k-1) given M(0..u-1).
 // Returns |C(u)|; writes C(u) (ascending) to out unless out == nullptr.
-// ---------------------------------------------------------------------------
 //
 // mode kFull: the plan's inputs -- C(dep) and N(M(b)) for b in BN_sh(u) under
 // result reuse, else N(M(b)) for b in BN(u).  kHoisted: H(u) (and
 // N(M(u-1)) when hoist = 1).  kBuildH: writes H(u), filtering only against
-// M(0..u-2).  Statistics always count the plan's inputs (the merge model),
-// whichever lists are read.
+// M(0..hfix(u)).  Statistics always count the plan's inputs (the merge
+// model), whichever lists are read.  The inputs are wave-uniform; their
+// descriptors live in the wave's LDS slot (S.ulp / S.uhb / S.uln), which
+// keeps them out of the vector registers.
+// ---------------------------------------------------------------------------
 template <bool STATS>
-__device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int u,
+__device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u,
                            int *__restrict__ out, const int *lbase, const int *hbase, int mode, bool count_stats,
                            unsigned long long &balg, unsigned long long &next) {
   const int lane = lane_id();
   const int ri = P.reuse ? 0 : 1;
   const int dep = P.reuse ? pl->dep[u] : -1;
-  const int *lp[7];
-  const unsigned *hb[7];
-  int ln[7];
-#pragma unroll
-  for (int i = 0; i < 7; i++) {
-    lp[i] = nullptr;
-    hb[i] = nullptr;
-    ln[i] = 0;
-  }
-  int nl = 0;
   unsigned long long bytes = 0;
   // labeled plans read only the label-l(u) segment of each adjacency list
   const int seg = (pl->labeled && P.colidx_lab) ? pl->label[u] : -1;
@@ -299,75 +295,48 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
       }
     }
   }
+  int nl = 0;
+  auto put_adj = [&](int b) {  // input nl: adjacency (segment) of M(b)
+    const int t = S.M[b];
+    int len;
+    const int *p = adj_list(P, t, seg, len);
+    S.ulp[nl] = p;
+    S.uhb[nl] = hub_row(P, t);
+    S.uln[nl] = len;
+    nl++;
+  };
+  auto put_list = [&](const int *p, int len) {  // input nl: a workspace list
+    S.ulp[nl] = p;
+    S.uhb[nl] = nullptr;
+    S.uln[nl] = len;
+    nl++;
+  };
   if (mode == kHoisted) {
-    lp[0] = hbase + (long long)(u - 3) * P.dstride;
-    ln[0] = S.hl[u];
-    nl = 1;
-    if (pl->hoist[ri][u] == 1) {
-      const int t = S.M[u - 1];
-      lp[1] = adj_list(P, t, seg, ln[1]);
-      hb[1] = hub_row(P, t);
-      nl = 2;
-    }
+    put_list(hbase + (long long)(u - 3) * P.dstride, S.hl[u]);
+    if (pl->hoist[ri][u] == 1) put_adj(u - 1);
   } else if (mode == kBuildH) {
+    if (dep >= 0) put_list(lbase + (long long)(dep - 2) * P.dstride, S.len[dep - 1]);
     const int nx = pl->hx_n[ri][u];
-    int off = 0;
-    if (dep >= 0) {
-      lp[0] = lbase + (long long)(dep - 2) * P.dstride;
-      ln[0] = S.len[dep - 1];
-      off = 1;
-    }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      if (i < nx) {
-        const int t = S.M[pl->hx[ri][u][i]];
-        if (off) {
-          lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
-          hb[i + 1] = hub_row(P, t);
-        } else {
-          lp[i] = adj_list(P, t, seg, ln[i]);
-          hb[i] = hub_row(P, t);
-        }
-      }
-    }
-    nl = off + nx;
+    for (int i = 0; i < nx; i++) put_adj(pl->hx[ri][u][i]);
   } else if (dep >= 0) {
-    lp[0] = lbase + (long long)(dep - 2) * P.dstride;  // C(dep) is the list of level dep-1
-    ln[0] = S.len[dep - 1];
+    put_list(lbase + (long long)(dep - 2) * P.dstride, S.len[dep - 1]);  // C(dep): list of level dep-1
     const int nsh = pl->num_bn_sh[u];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      if (i < nsh) {
-        const int t = S.M[pl->bn_sh[u][i]];
-        lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
-        hb[i + 1] = hub_row(P, t);
-      }
-    }
-    nl = 1 + nsh;
+    for (int i = 0; i < nsh; i++) put_adj(pl->bn_sh[u][i]);
   } else {
     const int nb = pl->num_bn[u];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      if (i < nb) {
-        const int t = S.M[pl->bn[u][i]];
-        lp[i] = adj_list(P, t, seg, ln[i]);
-        hb[i] = hub_row(P, t);
-      }
-    }
-    nl = nb;
+    for (int i = 0; i < nb; i++) put_adj(pl->bn[u][i]);
   }
   // probe = shortest list
-  const int *pp = lp[0];
-  int pn = ln[0];
   int pidx = 0;
-#pragma unroll
-  for (int i = 1; i < 7; i++) {
-    if (i < nl && ln[i] < pn) {
-      pp = lp[i];
-      pn = ln[i];
+  int pn = S.uln[0];
+  for (int i = 1; i < nl; i++) {
+    const int l = S.uln[i];
+    if (l < pn) {
+      pn = l;
       pidx = i;
     }
   }
+  const int *pp = S.ulp[pidx];
   const bool leaf = (u == pl->k - 1) && mode != kBuildH;
   const bool labeled = pl->labeled != 0;
   if (STATS && count_stats) {
@@ -429,9 +398,13 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
           for (int j = 0; j < 6; j++)
             if ((nemask >> j) & 1) ok = ok && (x != nev[j]);
           if (label_check && ok) ok = (int)P.labels[x] == lab;
-#pragma unroll
-          for (int i = 0; i < 7; i++)
-            if (i < nl && i != pidx && ok) ok = hb[i] ? bit_test(hb[i], x) : contains(lp[i], ln[i], x);
+#pragma unroll 1
+          for (int i = 0; i < nl; i++) {
+            if (i != pidx && ok) {
+              const unsigned *h = S.uhb[i];
+              ok = h ? bit_test(h, x) : contains(S.ulp[i], S.uln[i], x);
+            }
+          }
         }
         const unsigned long long mask = __ballot(ok);
         if (out) {
@@ -459,13 +432,16 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // Count-only leaf for elements [i0, i1) (at most 64) of stack level L = k-3:
 // element x fixes M(k-2) (for L = 0 the edge pair fixes M(0), M(1)) and
 // contributes |C(k-1)|.  Lane l owns element i0 + l and builds that
-// element's lists, window and probe; the probe candidates of all owners are
-// then flattened into one pair space (wave prefix scan) and each lane tests
-// one (owner, candidate) pair per step, finding its owner by a binary search
-// over the scan with ds_bpermute.  Same result and statistics as calling
-// extend() for each element in turn (with H(k-1) when it is built).
+// element's window and probe; the probe candidates of all owners are then
+// flattened into one pair space (wave prefix scan) and each lane tests one
+// (owner, candidate) pair per step, finding its owner by a binary search over
+// the scan with ds_bpermute.  Inputs that do not depend on the owner (C(dep),
+// H(u), N(M(b)) for b <= L) are wave-uniform and kept in LDS; at most two
+// depend on it (N(M(L+1)), or N(M(0)) and N(M(1)) at L = 0) and stay in
+// registers.  Same result and statistics as calling extend() for each element
+// in turn (with H(k-1) when it is built).
 template <bool STATS>
-__device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const DevPlan *__restrict__ pl, const WaveLds &S, int L,
+__device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int L,
                                          int i0, int i1, const int *L0r, const int *L0c, const int *lbase,
                                          const int *hbase, unsigned long long &balg, unsigned long long &next,
                                          int &taken) {
@@ -475,39 +451,28 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   const bool labeled = pl->labeled != 0;
   const int idx = i0 + lane;
   const bool act = idx < i1;
-  int pm[6];
-#pragma unroll
-  for (int j = 0; j < 6; j++) pm[j] = S.M[j];
+  // the owner's own vertices: M(L+1) for L > 0; M(0), M(1) for L = 0 (u = 2)
+  int xa = 0, xb = 0;
   if (L == 0) {
-    pm[0] = act ? L0r[idx] : 0;
-    pm[1] = act ? L0c[idx] : 0;
+    xa = act ? L0r[idx] : 0;
+    xb = act ? L0c[idx] : 0;
   } else {
-    const int x = act ? lbase[(long long)(L - 1) * P.dstride + idx] : 0;
-#pragma unroll
-    for (int j = 0; j < 6; j++)
-      if (j == L + 1) pm[j] = x;
+    xa = act ? lbase[(long long)(L - 1) * P.dstride + idx] : 0;
   }
-  auto pick = [&](int b) {
-    int t = 0;
-#pragma unroll
-    for (int j = 0; j < 6; j++)
-      if (b == j) t = pm[j];
-    return t;
+  auto own = [&](int b) { return L == 0 ? b <= 1 : b == L + 1; };
+  auto pick = [&](int b) -> int {  // by value: no address of xa / xb is formed
+    if (L == 0) {
+      if (b == 0) return xa;
+      return xb;
+    }
+    if (b == L + 1) return xa;
+    const int m = S.M[b];
+    return m;
   };
   const int ri = P.reuse ? 0 : 1;
   const int dep = P.reuse ? pl->dep[u] : -1;
   // H(u) serves every owner of the batch (L > 0: M(0..u-2) are shared)
   const bool hoisted = L > 0 && S.hv[u] != 0;
-  const int *lp[7];
-  const unsigned *hb[7];
-  int ln[7];
-#pragma unroll
-  for (int i = 0; i < 7; i++) {
-    lp[i] = nullptr;
-    hb[i] = nullptr;
-    ln[i] = 0;
-  }
-  int nl = 0;
   unsigned long long bytes = 0;
   const int seg = (labeled && P.colidx_lab) ? pl->label[u] : -1;
   int minfull = INT_MAX;
@@ -517,71 +482,88 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
       bytes += 4ull * (unsigned)l;
       minfull = l;
       const int nsh = pl->num_bn_sh[u];
-#pragma unroll
-      for (int i = 0; i < 6; i++) {
-        if (i < nsh) {
-          const int d = P.degv[pick(pl->bn_sh[u][i])];
-          bytes += 4ull * (unsigned)d + 16;
-          minfull = min(minfull, d);
-        }
+      for (int i = 0; i < nsh; i++) {
+        const int d = P.degv[pick(pl->bn_sh[u][i])];
+        bytes += 4ull * (unsigned)d + 16;
+        minfull = min(minfull, d);
       }
     } else {
       const int nb = pl->num_bn[u];
-#pragma unroll
-      for (int i = 0; i < 6; i++) {
-        if (i < nb) {
-          const int d = P.degv[pick(pl->bn[u][i])];
-          bytes += 4ull * (unsigned)d + 16;
-          minfull = min(minfull, d);
-        }
+      for (int i = 0; i < nb; i++) {
+        const int d = P.degv[pick(pl->bn[u][i])];
+        bytes += 4ull * (unsigned)d + 16;
+        minfull = min(minfull, d);
       }
     }
   }
-  if (hoisted) {
-    lp[0] = hbase + (long long)(u - 3) * P.dstride;
-    ln[0] = S.hl[u];
-    nl = 1;
-    if (pl->hoist[ri][u] == 1) {
-      const int t = pick(u - 1);
-      lp[1] = adj_list(P, t, seg, ln[1]);
-      hb[1] = hub_row(P, t);
-      nl = 2;
-    }
-  } else if (dep >= 0) {
-    lp[0] = lbase + (long long)(dep - 2) * P.dstride;
-    ln[0] = S.len[dep - 1];
-    const int nsh = pl->num_bn_sh[u];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      if (i < nsh) {
-        const int t = pick(pl->bn_sh[u][i]);
-        lp[i + 1] = adj_list(P, t, seg, ln[i + 1]);
-        hb[i + 1] = hub_row(P, t);
+  // inputs: nu wave-uniform ones in LDS; the owner's own in two fixed slots
+  // (A: N(M(L+1)), or N(M(0)) at L = 0; B: N(M(1)) at L = 0)
+  int nu = 0;
+  bool hasA = false, hasB = false;
+  const int *vp0 = nullptr, *vp1 = nullptr;
+  int vh0 = -1, vh1 = -1;  // hub bitmap rows (-1: none)
+  int vn0 = 0, vn1 = 0;
+  auto add_adj = [&](int b) {
+    const int t = pick(b);
+    int len;
+    const int *p = adj_list(P, t, seg, len);
+    if (own(b)) {
+      const int h = P.hub_idx[t];
+      if (L == 0 && b == 1) {
+        vp1 = p;
+        vh1 = h;
+        vn1 = len;
+        hasB = true;
+      } else {
+        vp0 = p;
+        vh0 = h;
+        vn0 = len;
+        hasA = true;
       }
+    } else {
+      S.ulp[nu] = p;
+      S.uhb[nu] = hub_row(P, t);
+      S.uln[nu] = len;
+      nu++;
     }
-    nl = 1 + nsh;
+  };
+  auto add_list = [&](const int *p, int len) {
+    S.ulp[nu] = p;
+    S.uhb[nu] = nullptr;
+    S.uln[nu] = len;
+    nu++;
+  };
+  if (hoisted) {
+    add_list(hbase + (long long)(u - 3) * P.dstride, S.hl[u]);
+    if (pl->hoist[ri][u] == 1) add_adj(u - 1);
+  } else if (dep >= 0) {
+    add_list(lbase + (long long)(dep - 2) * P.dstride, S.len[dep - 1]);
+    const int nsh = pl->num_bn_sh[u];
+    for (int i = 0; i < nsh; i++) add_adj(pl->bn_sh[u][i]);
   } else {
     const int nb = pl->num_bn[u];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      if (i < nb) {
-        const int t = pick(pl->bn[u][i]);
-        lp[i] = adj_list(P, t, seg, ln[i]);
-        hb[i] = hub_row(P, t);
-      }
-    }
-    nl = nb;
+    for (int i = 0; i < nb; i++) add_adj(pl->bn[u][i]);
   }
-  const int *pp = lp[0];
-  int pn = ln[0];
-  int pidx = 0;
-#pragma unroll
-  for (int i = 1; i < 7; i++) {
-    if (i < nl && ln[i] < pn) {
-      pp = lp[i];
-      pn = ln[i];
+  const int nl = nu + (hasA ? 1 : 0) + (hasB ? 1 : 0);
+  // probe = shortest input: index i < nu (uniform) or 8 / 9 (own)
+  int pidx = 0, pn = INT_MAX;
+  for (int i = 0; i < nu; i++) {
+    const int l = S.uln[i];
+    if (l < pn) {
+      pn = l;
       pidx = i;
     }
+  }
+  const int *pp = nu > 0 ? S.ulp[pidx] : nullptr;
+  if (hasA && vn0 < pn) {
+    pn = vn0;
+    pidx = 8;
+    pp = vp0;
+  }
+  if (hasB && vn1 < pn) {
+    pn = vn1;
+    pidx = 9;
+    pp = vp1;
   }
   if (STATS && labeled) bytes += (unsigned)minfull;
   const bool label_check = labeled && seg < 0;
@@ -594,7 +576,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   for (int j = 0; j < 6; j++) {
     nev[j] = -1;
     if (j < u) {
-      const int m = pm[j];
+      const int m = pick(j);
       if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
       else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
       else if ((same >> j) & 1) {
@@ -676,20 +658,29 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
 #pragma unroll
     for (int j = 0; j < 6; j++) {
       if ((nemask >> j) & 1) {
-        const int ne = __shfl(nev[j], o);
+        const int ne = own(j) ? __shfl(nev[j], o) : nev[j];
         ok = ok && (y != ne);
       }
     }
     if (label_check && ok) ok = (int)P.labels[y] == lab;
-#pragma unroll
-    for (int i = 0; i < 7; i++) {
-      if (i < nl) {
-        const int *lpo = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(lp[i]), o));
-        const unsigned *hbo =
-            reinterpret_cast<const unsigned *>(__shfl(reinterpret_cast<unsigned long long>(hb[i]), o));
-        const int lno = __shfl(ln[i], o);
-        if (i != po && ok) ok = hbo ? bit_test(hbo, y) : contains(lpo, lno, y);
+#pragma unroll 1
+    for (int i = 0; i < nu; i++) {
+      if (i != po && ok) {
+        const unsigned *h = S.uhb[i];
+        ok = h ? bit_test(h, y) : contains(S.ulp[i], S.uln[i], y);
       }
+    }
+    if (hasA) {
+      const int *p = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(vp0), o));
+      const int h = __shfl(vh0, o);
+      const int n = __shfl(vn0, o);
+      if (po != 8 && ok) ok = h >= 0 ? bit_test(P.hub_bits + (long long)h * P.hub_words, y) : contains(p, n, y);
+    }
+    if (hasB) {
+      const int *p = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(vp1), o));
+      const int h = __shfl(vh1, o);
+      const int n = __shfl(vn1, o);
+      if (po != 9 && ok) ok = h >= 0 ? bit_test(P.hub_bits + (long long)h * P.hub_words, y) : contains(p, n, y);
     }
     found += (unsigned)__popcll(__ballot(ok));
   }
@@ -995,7 +986,7 @@ template <bool STATS>
 __device__ __forceinline__ void match_body(const KParams &P) {
   __shared__ WaveLds lds[kWavesPerBlock];
   const int lane = lane_id();
-  const int wib = threadIdx.x / kWave;
+  const int wib = uni(threadIdx.x / kWave);  // wave-uniform: the workspace pointers stay scalar
   const long long gw = (long long)blockIdx.x * kWavesPerBlock + wib;
   int *wb = P.wbuf + gw * P.wstride;
   int *L0r = wb;
