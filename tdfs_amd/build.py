@@ -60,34 +60,20 @@ def build_lib() -> Path:
 
 
 def build_oracle() -> Path:
-    out = ORACLE / "liboracle.so"
-    _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-pthread", "-Wall",
-          str(ORACLE / "tdfs_oracle.c"), "-o", str(out)])
-    return out
+    """The oracle (test infrastructure), by its recipe oracle/Makefile."""
+    _run(["make", "-B", "-C", str(ORACLE), "liboracle.so"])
+    return ORACLE / "liboracle.so"
 
 
 def build_ref() -> Path | None:
     """bliss 0.73 compiled from the reference's own sources, plus the
-    oracle's driver that returns Aut(P) as bliss + closure computes it.
-    Skipped when the reference tree is absent."""
+    oracle's driver that returns Aut(P) as bliss + closure computes it
+    (recipe: oracle/Makefile, outputs only under oracle/_ref/).  Skipped when
+    the reference tree is absent."""
     if not REF_BLISS.is_dir():
         return None
-    refdir = ORACLE / "_ref"
-    refdir.mkdir(exist_ok=True)
-    srcs = ["defs.cc", "graph.cc", "partition.cc", "orbit.cc", "uintseqhash.cc", "heap.cc",
-            "timer.cc", "utils.cc"]
-    objs = []
-    for s in srcs:
-        o = refdir / (Path(s).stem + ".o")
-        _run(["g++", "-O2", "-fPIC", "-std=c++17", "-w", f"-I{REF_BLISS}", "-c", str(REF_BLISS / s),
-              "-o", str(o)])
-        objs.append(str(o))
-    out = refdir / "libbliss_ref.so"
-    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-shared", f"-I{REF_BLISS}",
-          str(ORACLE / "bliss_aut_ref.cpp"), *objs, "-o", str(out)])
-    for o in objs:
-        Path(o).unlink()
-    return out
+    _run(["make", "-B", "-j4", "-C", str(ORACLE), "_ref/libbliss_ref.so", f"REF_BLISS={REF_BLISS}"])
+    return ORACLE / "_ref" / "libbliss_ref.so"
 
 
 def build_all() -> None:
