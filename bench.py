@@ -32,7 +32,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # timed match kernel per tdfs_tuning.waves_per_simd
-KERNEL_NAMES = {0: "tdfs_match_kernel", 4: "tdfs_match_kernel_w4", 8: "tdfs_match_kernel_w8"}
+KERNEL_NAMES = {0: "tdfs_match_kernel", 4: "tdfs_match_kernel_w4", 5: "tdfs_match_kernel_w5",
+                6: "tdfs_match_kernel_w6", 8: "tdfs_match_kernel_w8"}
 
 
 def load_workload(name: str):

@@ -93,7 +93,7 @@ typedef struct tdfs_tuning {
   int32_t cursor_order;      /* level-0 order of a batch: 1 one cursor over (edge chunk, plan), all plans
                                 sweep the graph together (default); 2 one cursor per plan (serving) */
   int32_t waves_per_simd;    /* register budget of the timed kernel: 0 as compiled (default, no spills),
-                                4 or 8 waves per SIMD (spills) */
+                                4, 5, 6 or 8 waves per SIMD (spills) */
   int32_t hoist_min;         /* the inputs of C(u) that do not involve M(u-1) are intersected once per
                                 M(u-2) when the level holds at least this many candidates (default 2;
                                 < 0: never) */

@@ -1,7 +1,7 @@
 """Autotuner for the batched launch.
 
 Coordinate descent over the launch knobs of tdfs_tuning -- register budget
-(waves_per_simd: as compiled without spills, or 4 / 8 with), persistent-grid
+(waves_per_simd: as compiled without spills, or 4 / 5 / 6 / 8 with), persistent-grid
 occupancy (blocks_per_cu: waves resident per CU, i.e. how many DFS stacks
 share a CU's registers and LDS), level-0 chunk (chunk_edges: edges per cursor
 grab), donation threshold (timeout_us), donated piece size (split_chunk) and
@@ -24,7 +24,7 @@ import time
 from pathlib import Path
 
 SPACE = {
-    "waves_per_simd": [0, 4, 8],
+    "waves_per_simd": [0, 4, 5, 6, 8],
     "blocks_per_cu": [2, 3, 4],
     "chunk_edges": [16, 32, 64, 128],
     "timeout_us": [25, 100, 200, 1000],

@@ -1171,9 +1171,11 @@ __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
   match_body<STATS>(P);
 }
 
-// The same loop compiled for 4 or 8 waves per SIMD: more resident DFS stacks
-// to hide memory latency, at the cost of register spills.
+// The same loop compiled for 4, 5, 6 or 8 waves per SIMD: more resident DFS
+// stacks to hide memory latency, at the cost of register spills.
 __global__ void __launch_bounds__(kBlock, 4) tdfs_match_kernel_w4(KParams P) { match_body<false>(P); }
+__global__ void __launch_bounds__(kBlock, 5) tdfs_match_kernel_w5(KParams P) { match_body<false>(P); }
+__global__ void __launch_bounds__(kBlock, 6) tdfs_match_kernel_w6(KParams P) { match_body<false>(P); }
 __global__ void __launch_bounds__(kBlock, 8) tdfs_match_kernel_w8(KParams P) { match_body<false>(P); }
 
 // Sets bit y of row h for every neighbour y of hub vertex hubs[h].
@@ -1278,8 +1280,7 @@ struct tdfs_ctx {
   int wallclock_khz;
   int blocks_per_cu;        // occupancy limit of the timed kernel
   int blocks_per_cu_stats;  // and of the statistics kernel
-  int blocks_per_cu_w4;     // and of the timed kernel compiled for 4 waves per SIMD
-  int blocks_per_cu_w8;     // and for 8
+  int blocks_per_cu_w[9];   // and of the timed kernel compiled for w = 4, 5, 6, 8 waves per SIMD
   tdfs_tuning tune;
   int stats;
   Launch main;              // used by tdfs_count / tdfs_count_batch
@@ -1330,7 +1331,8 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->hub_budget_mb == 0) t->hub_budget_mb = d.hub_budget_mb;
   if (t->degree_order <= 0) t->degree_order = d.degree_order;
   if (t->cursor_order <= 0) t->cursor_order = d.cursor_order;
-  if (t->waves_per_simd != 4 && t->waves_per_simd != 8) t->waves_per_simd = 0;
+  if (t->waves_per_simd != 4 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
+    t->waves_per_simd = 0;
   if (t->leaf_pair_cap <= 0) t->leaf_pair_cap = d.leaf_pair_cap;
   if (t->leaf_pair_cap < 64) t->leaf_pair_cap = 64;
   if (t->hoist_min == 0) t->hoist_min = d.hoist_min;
@@ -1389,15 +1391,16 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
   c->wallclock_khz = khz > 0 ? khz : 100000;
-  int nb = 0, nb2 = 0, nb4 = 0, nb8 = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb4, tdfs_match_kernel_w4, kBlock, 0));
-  c->blocks_per_cu_w4 = std::max(1, nb4);
+  int nb = 0, nb2 = 0, nbw[9] = {};
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[4], tdfs_match_kernel_w4, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[5], tdfs_match_kernel_w5, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[6], tdfs_match_kernel_w6, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[8], tdfs_match_kernel_w8, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdfs_match_kernel<false>, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, tdfs_match_kernel<true>, kBlock, 0));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb8, tdfs_match_kernel_w8, kBlock, 0));
   c->blocks_per_cu = std::max(1, nb);
   c->blocks_per_cu_stats = std::max(1, nb2);
-  c->blocks_per_cu_w8 = std::max(1, nb8);
+  for (int w = 0; w < 9; w++) c->blocks_per_cu_w[w] = std::max(1, nbw[w]);
   int rc = launch_init(c->main, device);
   if (rc) {
     launch_free(c->main);
@@ -1805,7 +1808,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   const long long wstride = 2ll * kChunkMax + 2ll * (kMaxK - 3) * dstride;
   const int wv = stats ? 0 : t.waves_per_simd;
   const int occ = stats ? ctx->blocks_per_cu_stats
-                        : (wv == 8 ? ctx->blocks_per_cu_w8 : wv == 4 ? ctx->blocks_per_cu_w4 : ctx->blocks_per_cu);
+                        : (wv > 0 ? ctx->blocks_per_cu_w[wv] : ctx->blocks_per_cu);
   int bpc = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, occ) : occ;
   if (blocks_cap > 0) bpc = std::min(bpc, blocks_cap);
   const int grid = ctx->cus * bpc;
@@ -1875,6 +1878,10 @@ static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
     hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   else if (P.variant == 8)
     hipLaunchKernelGGL(tdfs_match_kernel_w8, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+  else if (P.variant == 6)
+    hipLaunchKernelGGL(tdfs_match_kernel_w6, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+  else if (P.variant == 5)
+    hipLaunchKernelGGL(tdfs_match_kernel_w5, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   else if (P.variant == 4)
     hipLaunchKernelGGL(tdfs_match_kernel_w4, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   else

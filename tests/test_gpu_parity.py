@@ -136,6 +136,8 @@ def test_scheduling_modes_are_exact(dev):
         dict(cursor_order=2),
         dict(waves_per_simd=8),
         dict(waves_per_simd=4, timeout_us=1),
+        dict(waves_per_simd=5),
+        dict(waves_per_simd=6, timeout_us=1),
         dict(leaf_pair_cap=64, timeout_us=1),
         dict(leaf_pair_cap=100000),
         dict(cursor_order=2, timeout_us=1, chunk_edges=16),
