@@ -213,6 +213,19 @@ int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_
 int tdfs_count_batch(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
                      tdfs_result *out);
 
+/* Embedding output (no counterpart in the reference, which only counts:
+ * src/gpu_match.cu:331,409,933).  Runs the counting DFS of tdfs_count and
+ * writes every embedding it reaches as a row of k vertex ids (M(0), ..,
+ * M(k-1), pattern vertices in plan order, ids in the caller's numbering) to
+ * rows[0 .. cap*k); the order of rows is unspecified.  Unlabeled plans yield
+ * one embedding per automorphism class (the symmetry-broken raw set, whose
+ * size times |Aut(P)| is the count); labeled plans every label-preserving
+ * one.  *written = min(raw, cap); out->raw / out->count are complete even
+ * when rows were cut at cap.  Allocates a device buffer of cap rows per call:
+ * meant for small outputs (checks, examples). */
+int tdfs_enumerate(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin, int64_t edge_end,
+                   int32_t *rows, uint64_t cap, uint64_t *written, tdfs_result *out);
+
 /* ---------------- prepared batches (hipGraph) ---------------- */
 typedef struct tdfs_batch tdfs_batch;
 /* Uploads the plans once and captures the whole launch sequence (reset

@@ -59,6 +59,9 @@ def lib():
         L.orc_prepare.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]
         L.orc_release.argtypes = [C.c_void_p]
+        L.orc_enumerate.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32), C.POINTER(OrcPlan), C.POINTER(C.c_int32), C.c_uint64,
+                                    C.POINTER(C.c_uint64)]
         L.orc_count_prepared.argtypes = [C.c_void_p, C.POINTER(OrcPlan), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int64), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
         L.orc_bruteforce_count.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
@@ -109,6 +112,31 @@ def count(rowptr, colidx, labels, p: OrcPlan, edge_lo: int = 0, edge_hi: int = -
         "extensions": int(sum(out[2:9])),
         "nodes": [int(x) for x in out[2:9]],
     }
+
+
+def enumerate_embeddings(rowptr, colidx, labels, p: OrcPlan, cap: int = -1) -> tuple[np.ndarray, int]:
+    """Every embedding the counting DFS reaches, as rows (M(0), .., M(k-1)) in
+    plan order (at most cap rows; -1: all), and the raw count."""
+    rp, ci, lb = _ptrs(rowptr, colidx, labels)
+    n = rp.shape[0] - 1
+    k = int(p.k)
+    total = C.c_uint64()
+    if cap < 0:
+        rc = lib().orc_enumerate(n, ci.shape[0], rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                 ci.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 None if lb is None else lb.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(p), None,
+                                 0, C.byref(total))
+        if rc:
+            raise ValueError(f"oracle enumerate failed ({rc})")
+        cap = int(total.value)
+    out = np.zeros((max(cap, 1), k), dtype=np.int32)
+    rc = lib().orc_enumerate(n, ci.shape[0], rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                             ci.ctypes.data_as(C.POINTER(C.c_int32)),
+                             None if lb is None else lb.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(p),
+                             out.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(total))
+    if rc:
+        raise ValueError(f"oracle enumerate failed ({rc})")
+    return out[: min(cap, int(total.value))], int(total.value)
 
 
 def count_ranges(rowptr, colidx, labels, p: OrcPlan, los, his, threads: int = 1, reuse: bool = True) -> dict:

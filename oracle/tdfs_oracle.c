@@ -307,6 +307,8 @@ typedef struct {
   uint64_t count;
   uint64_t balg;
   uint64_t nodes[ORC_MAXK]; /* extensions performed per pattern vertex u */
+  int32_t *emit;            /* enumeration: k ids per embedding (NULL: count only) */
+  uint64_t emit_cap, emitted;
 } orc_worker;
 
 static inline int64_t deg_of(const orc_graph *g, int32_t v) { return g->rowptr[v + 1] - g->rowptr[v]; }
@@ -384,6 +386,14 @@ static int64_t extend(orc_worker *w, int u) {
     for (int i = 0; i < nl && ok; i++) if (i != pi) ok = contains(lists[i], sizes[i], x);
     if (!ok) continue;
     if (!leaf) out[n] = x;
+    if (leaf && w->emit) { /* embedding (M(0..k-2), x), in plan order */
+      if (w->emitted < w->emit_cap) {
+        int32_t *e = w->emit + w->emitted * (uint64_t)p->k;
+        for (int j = 0; j < u; j++) e[j] = w->M[j];
+        e[u] = x;
+      }
+      w->emitted++;
+    }
     n++;
   }
   if (!leaf) bytes += 4ull * (uint64_t)n;
@@ -574,6 +584,29 @@ int orc_count(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *coli
               int nthreads, int reuse, uint64_t *out) {
   return orc_count_ranges(n, nnz, rowptr, colidx, labels, p, &edge_lo, &edge_hi, 1, nthreads, reuse,
                           out);
+}
+
+/* Enumeration (the counting DFS, single-threaded, writing every embedding):
+ * out receives up to cap embeddings of k vertex ids each, in plan order
+ * (M(0..k-1)), in the DFS's edge order; *total = the raw count (all
+ * embeddings, also those beyond cap).  Returns 0, or a negative code. */
+int orc_enumerate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
+                  const int32_t *labels, const orc_plan *p, int32_t *out, uint64_t cap, uint64_t *total) {
+  if (p->labeled && !labels) return -1;
+  orc_graph *g = (orc_graph *)orc_prepare(n, nnz, rowptr, colidx, labels);
+  orc_worker w;
+  memset(&w, 0, sizeof(w));
+  w.g = g;
+  w.p = p;
+  w.reuse = 1;
+  w.emit = out;
+  w.emit_cap = out ? cap : 0;
+  for (int u = 0; u < ORC_MAXK; u++) w.cand[u] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(g->dmax > 0 ? g->dmax : 1));
+  for (int64_t e = 0; e < nnz; e++) run_edge(&w, e);
+  for (int u = 0; u < ORC_MAXK; u++) free(w.cand[u]);
+  *total = w.count;
+  orc_release(g);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -282,6 +282,20 @@ class Device:
         check(lib().tdfs_count_batch(self.handle, dg.handle, arr, len(plans), res), "count batch")
         return list(res)
 
+    def enumerate(self, dg: DeviceGraph, plan: Plan, cap: int, edge_begin: int = 0,
+                  edge_end: int = -1) -> tuple[np.ndarray, Result]:
+        """Embeddings reached by the counting DFS (tdfs_enumerate): an int32
+        array of min(raw, cap) rows of k vertex ids (plan order, caller's
+        numbering, unspecified row order) and the launch's Result."""
+        k = int(plan.k)
+        rows = np.zeros((max(cap, 1), k), dtype=np.int32)
+        written = C.c_uint64()
+        r = Result()
+        check(lib().tdfs_enumerate(self.handle, dg.handle, C.byref(plan), edge_begin, edge_end,
+                                   rows.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(written), C.byref(r)),
+              "enumerate")
+        return rows[: int(written.value)], r
+
     def close(self) -> None:
         if self.handle:
             lib().tdfs_ctx_destroy(self.handle)

@@ -151,6 +151,10 @@ struct KParams {
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
   int hoist_min;             // build H(u) for a level of at least this many candidates (0: never)
+  // enumeration (null emit: count only): embeddings as rows of k ids
+  int *emit;
+  unsigned long long emit_cap;  // rows
+  unsigned long long *emit_n;   // rows produced (also past emit_cap)
 };
 
 struct WaveLds {
@@ -250,6 +254,23 @@ __device__ __forceinline__ const int *adj_list(const KParams &P, int t, int lab,
   }
   len = P.degv[t];
   return P.colidx + P.rowptr[t];
+}
+
+// Writes the leaf embeddings (M(0..u-1), x) of the lanes set in `mask` to
+// consecutive rows of the enumeration buffer (one atomic per wave).
+__device__ __forceinline__ void emit_rows(const KParams &P, const WaveLds &S, int u, int x, bool ok,
+                                          unsigned long long mask) {
+  const unsigned c = (unsigned)__popcll(mask);
+  if (c == 0) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(P.emit_n, (unsigned long long)c);
+  base = __shfl(base, 0);
+  const unsigned long long pos = base + mbcnt64(mask);
+  if (ok && pos < P.emit_cap) {
+    int *row = P.emit + pos * (unsigned long long)(u + 1);
+    for (int j = 0; j < u; j++) row[j] = S.M[j];
+    row[u] = x;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -372,7 +393,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
   unsigned n = 0;
   if (start < stop) {
-    if (leaf && nl == 1 && label_known) {
+    if (leaf && nl == 1 && label_known && !P.emit) {
       // Count-only leaf over one list: |list ∩ (lo,hi)| minus the excluded
       // path vertices that lie in that window and in the list.
       n = (unsigned)(stop - start);
@@ -410,6 +431,8 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         if (out) {
           const unsigned pos = n + mbcnt64(mask);
           if (ok) out[pos] = x;
+        } else if (leaf && P.emit) {
+          emit_rows(P, S, u, x, ok, mask);
         }
         n += (unsigned)__popcll(mask);
       }
@@ -1101,6 +1124,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
     if (expired) {
       if (lane == 0) atomicOr(&P.stats[pid * 4 + 3], 1ull);  // dropped: count incomplete
     } else if (!pl->unsat) {
+      int floor = -1;  // stack level the DFS starts from (one call site for both kinds of work)
       if (kind == 1) {
         const long long tp = STATS ? clock64() : 0;
         const int n0 = filter_edges<STATS>(P, pl, e0, e1, L0r, L0c, balg);
@@ -1111,7 +1135,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           S.len[0] = n0;
           S.end[0] = n0;
           S.it[0] = 0;
-          dfs<STATS>(P, pl, S, pid, 0, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
+          floor = 0;
         }
       } else {
         const int p = t.p;
@@ -1119,28 +1143,28 @@ __device__ __forceinline__ void match_body(const KParams &P) {
         for (int j = 0; j < 8; j++)
           if (j < p) S.M[j] = t.M[j];
         const int k = pl->k;
-        unsigned long long dummy_b = 0, dummy_n = 0;
+        const bool fresh = t.fresh != 0;
         const long long tp = STATS ? clock64() : 0;
-        for (int u = 2; u < p; u++) {
-          const int n = (int)extend<STATS>(P, pl, S, u, lbase + (long long)(u - 2) * P.dstride, lbase, hbase, kFull,
-                                           false, dummy_b, dummy_n);
-          S.len[u - 1] = n;
+        // rebuild C(2..p-1) (the donor counted them), then C(p), counted in
+        // the statistics only when the donor never performed it (fresh)
+        for (int u = 2; u <= p; u++) {
+          const bool last = u == p;
+          const bool leaf = last && p == k - 1;
+          int *o = leaf ? nullptr : lbase + (long long)(u - 2) * P.dstride;
+          const unsigned n = extend<STATS>(P, pl, S, u, o, lbase, hbase, kFull, last && fresh, balg, next);
+          if (leaf) cnt += n;
+          else S.len[u - 1] = (int)n;
         }
         if (STATS) prof[4] += clock64() - tp;
-        const bool fresh = t.fresh != 0;
-        if (p == k - 1) {
-          cnt += extend<STATS>(P, pl, S, p, nullptr, lbase, hbase, kFull, fresh, balg, next);
-        } else {
-          const int n = (int)extend<STATS>(P, pl, S, p, lbase + (long long)(p - 2) * P.dstride, lbase, hbase, kFull,
-                                           fresh, balg, next);
-          S.len[p - 1] = n;
+        if (p < k - 1) {
+          const int n = S.len[p - 1];
           S.it[p - 1] = t.a;
           S.end[p - 1] = min(t.b, n);
           clear_hoists(S);
-          if (n > 0 && prepare_hoists<STATS>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase))
-            dfs<STATS>(P, pl, S, pid, p - 1, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
+          if (n > 0 && prepare_hoists<STATS>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase)) floor = p - 1;
         }
       }
+      if (floor >= 0) dfs<STATS>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
     }
     if (lane == 0) {
       if (cnt) atomicAdd(&P.counts[pid], cnt);
@@ -2203,4 +2227,65 @@ extern "C" int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *p
 extern "C" int tdfs_count_batch(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,
                                 tdfs_result *out) {
   return run_plans(ctx, g, plans, nplans, -1, -1, out);
+}
+
+extern "C" int tdfs_enumerate(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
+                              int64_t edge_end, int32_t *rows, uint64_t cap, uint64_t *written, tdfs_result *out) {
+  if (!ctx || !g || !plan || !written || !out || (cap > 0 && !rows)) return fail(TDFS_E_ARG, "bad arguments");
+  const int k = plan->k;
+  if (cap > (uint64_t)1 << 40) return fail(TDFS_E_ARG, "cap %llu too large", (unsigned long long)cap);
+  Launch &L = ctx->main;
+  KParams P;
+  int rc = launch_prepare(ctx, L, g, plan, 1, edge_begin, edge_end, false, 0, P);
+  if (rc) return rc;
+  int *d_rows = nullptr;
+  unsigned long long *d_n = nullptr;
+  auto release = [&]() {
+    if (d_rows) (void)hipFree(d_rows);
+    if (d_n) (void)hipFree(d_n);
+  };
+  if (cap > 0 && hipMalloc(&d_rows, sizeof(int) * (size_t)cap * (size_t)k) != hipSuccess) {
+    release();
+    return fail(TDFS_E_OOM, "enumeration buffer of %llu rows", (unsigned long long)cap);
+  }
+  if (hipMalloc(&d_n, sizeof(unsigned long long)) != hipSuccess) {
+    release();
+    return fail(TDFS_E_OOM, "enumeration counter");
+  }
+  P.emit = d_rows ? d_rows : reinterpret_cast<int *>(d_n);  // non-null marks enumeration
+  P.emit_cap = cap;
+  P.emit_n = d_n;
+  P.leaf_batch = 0;  // leaves are walked one parent at a time to write their rows
+  hipError_t e = hipMemsetAsync(d_n, 0, sizeof(unsigned long long), L.stream);
+  if (e == hipSuccess) e = hipEventRecord(L.ev0, L.stream);
+  if (e != hipSuccess) {
+    release();
+    return fail(TDFS_E_HIP, "enumerate: %s", hipGetErrorString(e));
+  }
+  rc = launch_enqueue(L, P, false);
+  if (rc) {
+    release();
+    return rc;
+  }
+  unsigned long long produced = 0;
+  e = hipEventRecord(L.ev1, L.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&produced, d_n, sizeof(produced), hipMemcpyDeviceToHost, L.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
+  const uint64_t nw = std::min<uint64_t>(produced, cap);
+  if (e == hipSuccess && nw > 0)
+    e = hipMemcpy(rows, d_rows, sizeof(int) * (size_t)nw * (size_t)k, hipMemcpyDeviceToHost);
+  release();
+  if (e != hipSuccess) return fail(TDFS_E_HIP, "enumerate: %s", hipGetErrorString(e));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, L.ev0, L.ev1));
+  rc = launch_collect(L, plan, 1, ms, out, nullptr);
+  if (rc) return rc;
+  // rows are in the numbering counted on: map the degree-ordered copy back
+  if (g->ordered && prune_is_sound(*plan)) {
+    std::vector<int32_t> inv(g->n);
+    for (int32_t v = 0; v < g->n; v++) inv[g->perm[v]] = v;
+    for (uint64_t i = 0; i < nw * (uint64_t)k; i++) rows[i] = inv[rows[i]];
+  }
+  *written = nw;
+  return TDFS_OK;
 }

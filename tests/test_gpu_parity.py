@@ -199,6 +199,41 @@ def test_dense_labeled_patterns_hoisted(dev):
     dg.free()
 
 
+def test_enumeration_matches_oracle(dev):
+    """tdfs_enumerate writes exactly the embeddings the oracle's DFS reaches
+    on the numbering the device counted on, mapped back to the caller's ids
+    (for unlabeled plans the symmetry-broken representatives depend on the
+    numbering); also with donation and hoisting forced.  Rows are cut at cap
+    while the count stays complete."""
+    G = T.CSRGraph.chung_lu(2000, 10000, 2.1, 200, 81).with_labels(3, 8)
+    dg = dev.upload(G)
+    base_t = dev.tuning()
+    for i in (1, 2, 4, 5, 7, 11):
+        text = fixture(i)
+        for labeled in (0, 1):
+            p = T.plan_pattern(text, labeled)
+            q = O.plan(text, labeled)
+            H = device_numbering(dg, G, p)
+            want, total = O.enumerate_embeddings(H.rowptr, H.colidx, H.labels if labeled else None, q)
+            if H is not G:
+                inv = np.empty(G.n, dtype=np.int64)
+                inv[dg.order()] = np.arange(G.n)
+                want = inv[want]
+            want = sorted(map(tuple, want.tolist()))
+            for mode in (dict(), dict(timeout_us=1, donate_levels=5, hoist_min=1)):
+                say("enumerate", f"{i}.g", labeled, mode)
+                dev.set_tuning(**mode)
+                rows, r = dev.enumerate(dg, p, cap=total + 5)
+                dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_ if f != "reserved"})
+                assert r.raw == total, (i, labeled, mode)
+                assert sorted(map(tuple, rows.tolist())) == want, (i, labeled, mode)
+            if total > 3:
+                rows, r = dev.enumerate(dg, p, cap=3)
+                assert rows.shape[0] == 3 and r.raw == total
+                assert set(map(tuple, rows.tolist())) <= set(want)
+    dg.free()
+
+
 def test_hub_beyond_reference_buffer(dev):
     """A hub of degree 6000 > 4096 (the reference's per-level slot size,
     src/config.h:12): lists are never truncated here."""

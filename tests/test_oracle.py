@@ -272,3 +272,55 @@ def test_counts_invariant_under_renumbering():
             counts = [O.count(X.rowptr, X.colidx, X.labels if labeled else None, p, threads=4)["raw"]
                       for X in (G, H, D)]
             assert counts[0] == counts[1] == counts[2], (text, labeled)
+
+
+def _all_monomorphisms(G, k, padj, plab, labeled):
+    """Every injective map of the pattern into G preserving edges (and labels
+    when labeled), by plain backtracking over vertex ids: independent of the
+    plan, its conditions and its pruning."""
+    nbr = [set(G.colidx[G.rowptr[v]:G.rowptr[v + 1]].tolist()) for v in range(G.n)]
+    out = []
+
+    def rec(m):
+        d = len(m)
+        if d == k:
+            out.append(tuple(m))
+            return
+        for v in range(G.n):
+            if v in m or (labeled and G.labels[v] != plab[d]):
+                continue
+            if all(v in nbr[m[j]] for j in range(d) if padj[d][j]):
+                rec(m + [v])
+    rec([])
+    return out
+
+
+@pytest.mark.parametrize("i", [1, 2, 3, 4, 5, 8, 9])
+def test_enumeration_is_the_counted_set(i):
+    """The oracle's enumeration (the counting DFS writing every embedding)
+    holds exactly raw distinct embeddings; unlabeled, the images of those
+    embeddings under Aut(P) are every monomorphism exactly once (so the
+    symmetry-broken set is one representative per automorphism class);
+    labeled, they are every label-preserving monomorphism."""
+    G = T.CSRGraph.erdos_renyi(24, 110, 30 + i).with_labels(2, i)
+    text = fixture(i)
+    for labeled in (0, 1):
+        p = O.plan(text, labeled)
+        k = int(p.k)
+        padj = [[int(p.adj[a][b]) for b in range(k)] for a in range(k)]
+        plab = [int(p.vlabel[a]) for a in range(k)]
+        rows, total = O.enumerate_embeddings(G.rowptr, G.colidx, G.labels if labeled else None, p)
+        assert total == O.count(G.rowptr, G.colidx, G.labels if labeled else None, p)["raw"]
+        got = [tuple(r) for r in rows.tolist()]
+        assert len(got) == len(set(got)) == total
+        every = _all_monomorphisms(G, k, padj, plab, labeled)
+        if labeled:
+            assert sorted(got) == sorted(every)
+        else:
+            auts = [s for s in itertools.permutations(range(k))
+                    if all(padj[s[a]][s[b]] == padj[a][b] for a in range(k) for b in range(k))]
+            assert len(auts) == p.mult
+            # m o sigma for every representative m and automorphism sigma
+            images = [tuple(m[s[a]] for a in range(k)) for m in got for s in auts]
+            assert len(images) == len(set(images))
+            assert set(images) == set(every)
