@@ -163,8 +163,9 @@ def main() -> int:
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: T-DFS shards level-0 edges across GPUs with a single "
-              "8-byte count reduction per pattern, which this build does not run (see DESIGN.md).",
+        print("bench.py measures one GPU only. Across GPUs, T-DFS would shard level-0 edge chunks "
+              "(tdfs_count's [edge_begin, edge_end) range, chunks interleaved over GPUs, graph replicated) "
+              "and finish with one 8-byte sum per pattern; that multi-GPU driver is not part of this library.",
               file=sys.stderr)
         return 2
 
