@@ -568,6 +568,61 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < nb; i++) add_adj(pl->bn[u][i]);
   }
   const int nl = nu + (hasA ? 1 : 0) + (hasB ? 1 : 0);
+  if (L > 0 && nu == 1 && hasA && !hasB && S.uln[0] <= kWave) {
+    // One shared input U (C(dep), H(u) or N(M(b)), b <= L) of at most 64
+    // elements and the owner's N(M(L+1)).  U is the probe for every owner
+    // (set semantics: the probe does not change C(u)): lane j holds U[j mod
+    // g], filtered once against everything but the owner's vertex, and 64 / g
+    // owners are tested per step (g = |U| rounded up to a power of two) -- no
+    // pair scan, no owner search.
+    const int *U = S.ulp[0];
+    const int nU = S.uln[0];
+    taken = i1 - i0;
+    if (STATS) {
+      if (labeled) bytes += (unsigned)minfull;
+      balg += wave_sum(act ? bytes : 0ull);
+      next += (unsigned)__popcll(__ballot(act));
+    }
+    const int ju = u - 1;  // the owner's vertex
+    const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
+    const int same = (labeled && seg >= 0) ? pl->same_label_mask[u] : 0x7f;
+    const bool c_gt = !labeled && ((gt >> ju) & 1);
+    const bool c_lt = !labeled && ((lt >> ju) & 1);
+    const bool c_ne = !c_gt && !c_lt && ((same >> ju) & 1);
+    int g = 1;
+    while (g < nU) g <<= 1;
+    const int per = kWave / g;
+    const int j = lane & (g - 1);
+    const int slot = lane / g;
+    const int y = j < nU ? U[j] : 0;
+    bool yv = j < nU;
+#pragma unroll 1
+    for (int i = 0; i < ju; i++) {
+      const int m = S.M[i];
+      if (!labeled && ((gt >> i) & 1)) yv = yv && y > m;
+      else if (!labeled && ((lt >> i) & 1)) yv = yv && y < m;
+      else if ((same >> i) & 1) yv = yv && y != m;
+    }
+    if (labeled && seg < 0 && yv) yv = (int)P.labels[y] == pl->label[u];
+    const int owners = i1 - i0;
+    unsigned long long found = 0;
+    for (int o0 = 0; o0 < owners; o0 += per) {
+      const int o = o0 + slot;
+      const bool ov = o < owners;
+      const int oo = ov ? o : 0;
+      const int x = __shfl(xa, oo);
+      const int h = __shfl(vh0, oo);
+      const int *lp = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(vp0), oo));
+      const int ln = __shfl(vn0, oo);
+      bool ok = yv && ov;
+      if (c_gt) ok = ok && y > x;
+      if (c_lt) ok = ok && y < x;
+      if (c_ne) ok = ok && y != x;
+      if (ok) ok = h >= 0 ? bit_test(P.hub_bits + (long long)h * P.hub_words, y) : contains(lp, ln, y);
+      found += (unsigned)__popcll(__ballot(ok));
+    }
+    return found;
+  }
   // probe = shortest input: index i < nu (uniform) or 8 / 9 (own)
   int pidx = 0, pn = INT_MAX;
   for (int i = 0; i < nu; i++) {
