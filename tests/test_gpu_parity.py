@@ -234,6 +234,34 @@ def test_enumeration_matches_oracle(dev):
     dg.free()
 
 
+def test_dense_core_long_shared_lists(dev):
+    """A complete core (K_110) inside a sparse graph: shared candidate lists
+    of 100+ elements next to owner lists of similar length (flattened pair
+    path; also with leaf_pair_cap = 64 and forced hoisting); counts and
+    statistics equal the oracle's, and the 4-clique count is at least the
+    closed form n!/(n-4)! of the core."""
+    core = 110
+    edges = [(a, b) for a in range(core) for b in range(a + 1, core)]
+    rng = np.random.default_rng(5)
+    n = 3000
+    for _ in range(6000):
+        a, b = rng.integers(core, n, size=2)
+        edges.append((int(a), int(b)))
+    G = T.CSRGraph.from_edges(n, edges).with_labels(2, 3)
+    dg = dev.upload(G)
+    base_t = dev.tuning()
+    k4 = T.plan_pattern(fixture(2), False)
+    r = dev.count(dg, k4)
+    assert r.count >= core * (core - 1) * (core - 2) * (core - 3)
+    for i in (1, 2, 7):
+        for labeled in (0, 1):
+            for mode in (dict(), dict(leaf_pair_cap=64, timeout_us=1), dict(hoist_min=1)):
+                say("dense core", f"{i}.g", labeled, mode)
+                check(dev, dg, G, fixture(i), labeled, **mode)
+                dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_ if f != "reserved"})
+    dg.free()
+
+
 def test_hub_beyond_reference_buffer(dev):
     """A hub of degree 6000 > 4096 (the reference's per-level slot size,
     src/config.h:12): lists are never truncated here."""
