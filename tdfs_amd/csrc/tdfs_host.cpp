@@ -397,8 +397,8 @@ struct Estimator {
 
   void dfs(int u) {
     if (capped) return;
-    const int32_t *lp[TDFS_MAX_K];
-    int64_t ln[TDFS_MAX_K];
+    const int32_t *lp[TDFS_MAX_K] = {};
+    int64_t ln[TDFS_MAX_K] = {};  // every vertex u >= 1 has an input (checked plans)
     int nl = 0;
     const int dep = p->shared_lvl[u];
     if (dep >= 2) {
