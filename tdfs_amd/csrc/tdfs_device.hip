@@ -15,11 +15,20 @@
 //     (src/gpu_match.cu:260-494).  The probe is the shortest list; the
 //     symmetry-breaking bounds narrow it by binary search before any lane work.
 //   * Result reuse: C(u) starts from C(shared_lvl[u]) (src/gpu_match.cu:733-776).
+//   * Hoisted intersections: the inputs of C(u) that do not involve M(u-1)
+//     are intersected into H(u) once the vertices they depend on are fixed,
+//     and shared by every candidate below.
+//   * Leaf: counted for 64 parents at once (flattened (parent, candidate)
+//     pairs, or one shared short list probed against every parent's list).
+//   * Labeled graphs keep each adjacency list sorted by (label, id), so an
+//     extension reads only the label-l(u) segment; hubs (degree >= 256) get
+//     neighbour bitmaps; a degree-ordered copy of the graph is counted on
+//     whenever the count cannot depend on the numbering.
 //   * Load balance: a wave that has held one task longer than the timeout while
 //     other waves idle splits the remaining iterations of its shallowest open
 //     level into tasks on an HBM ring (src/gpu_match.cu:863-908, src/queue.h);
 //     a receiver rebuilds the prefix's candidate lists, so reuse stays exact.
-//   * Termination: a wave leaves only when the cursor is spent, the ring is
+//   * Termination: a wave leaves only when the cursors are spent, the ring is
 //     empty and no wave holds work.
 #include <hip/hip_runtime.h>
 
