@@ -87,6 +87,26 @@ def cpu_baseline(g, pats, threads: int, budget_s: float = 15.0) -> dict:
     }
 
 
+_HIP = None
+
+
+def device_sync() -> None:
+    """Whole-device synchronisation around the timed region (every stream of
+    the library included): hipDeviceSynchronize from the HIP runtime the
+    library already loaded (torch.cuda.synchronize() without importing torch)."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+
+        try:  # the soname libtdfs.so links, so the already-loaded runtime is reused
+            _HIP = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            _HIP = ctypes.CDLL("libamdhip64.so", mode=ctypes.RTLD_GLOBAL)
+    rc = _HIP.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
 def measure(dev, dg, pats, steps: int, warmup: int, plain_too: bool = True) -> dict:
     """Exact counts + algorithmic bytes (statistics launch), then timed
     batched (hipGraph replay) and plain (one launch per pattern) steps."""
@@ -121,15 +141,18 @@ def measure(dev, dg, pats, steps: int, warmup: int, plain_too: bool = True) -> d
         if plain_too:
             plain()
     kb, kp = [], []
+    device_sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         kb.append(batched())
+    device_sync()
     t_b = (time.perf_counter() - t0) / steps
     t_p = float("nan")
     if plain_too:
         t0 = time.perf_counter()
         for _ in range(steps):
             kp.append(plain())
+        device_sync()
         t_p = (time.perf_counter() - t0) / steps
     batch.close()
     total = sum(counts)

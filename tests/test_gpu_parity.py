@@ -262,6 +262,29 @@ def test_dense_core_long_shared_lists(dev):
     dg.free()
 
 
+def test_cli_drivers_match_oracle(dev, tmp_path):
+    """The reference-compatible drivers read the binary graph files written
+    by tdfs_graph_write and end stdout with "<pattern>\t<ms>\t<count>"
+    (cu_test.cu:107): ulb prints raw x |Aut|, lb the label-preserving count,
+    both equal to the oracle's."""
+    import subprocess
+    G = T.CSRGraph.chung_lu(5000, 25000, 2.1, 400, 91).with_labels(3, 9)
+    prefix = tmp_path / "g"
+    G.write_bin(str(prefix))
+    for i in (1, 2, 4, 7):
+        pat = GOLDEN / "patterns" / f"{i}.g"
+        text = pat.read_text()
+        for name, labeled in (("ulb", 0), ("lb", 1)):
+            say("cli", name, f"{i}.g")
+            exe = Path(T.__file__).resolve().parent / "bin" / name
+            r = subprocess.run([str(exe), str(prefix), str(pat)], capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            last = r.stdout.strip().splitlines()[-1].split("\t")
+            assert last[0] == str(pat) and float(last[1]) >= 0.0
+            want = O.count(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(text, labeled), threads=4)
+            assert int(last[2]) == want["count"], (name, i)
+
+
 def test_hub_beyond_reference_buffer(dev):
     """A hub of degree 6000 > 4096 (the reference's per-level slot size,
     src/config.h:12): lists are never truncated here."""
