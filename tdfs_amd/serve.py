@@ -149,6 +149,8 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--blocks-per-cu", type=int, default=1)
+    ap.add_argument("--waves-per-simd", type=int, default=0,
+                    help="register budget of the match kernel (tdfs_tuning.waves_per_simd: 0, 4, 5, 6 or 8)")
     ap.add_argument("--max-batch", type=int, default=8)
     ap.add_argument("--budget", type=float, default=2e10)
     ap.add_argument("--check", type=int, default=20, help="queries recounted alone for equality")
@@ -171,8 +173,8 @@ def main() -> int:
     print(f"{len(plans)} queries {time.time() - t:.1f} s", file=sys.stderr, flush=True)
     out = {"workload": "c5: Chung-Lu n=4M m=34M gamma=2.3 seed 5, 8 labels; seeded query stream (k 3-7, 30% labeled)",
            "budget": args.budget, "slots": args.slots, "blocks_per_cu": args.blocks_per_cu,
-           "max_batch": args.max_batch, "streams": []}
-    dev.set_tuning(time_limit_ms=args.time_limit_ms)
+           "max_batch": args.max_batch, "waves_per_simd": args.waves_per_simd, "streams": []}
+    dev.set_tuning(time_limit_ms=args.time_limit_ms, waves_per_simd=args.waves_per_simd)
     server = Server(dev, dg, args.slots, args.blocks_per_cu, args.max_batch, args.budget)
     alone = {}
     for i in range(min(args.check, len(plans))):
