@@ -324,3 +324,74 @@ def test_enumeration_is_the_counted_set(i):
             images = [tuple(m[s[a]] for a in range(k)) for m in got for s in auts]
             assert len(images) == len(set(images))
             assert set(images) == set(every)
+
+
+def test_pins_catch_planted_mistakes():
+    """The checks are sensitive: plausible planner / DFS mistakes planted in
+    a copy of the oracle's plan (a dropped or flipped symmetry condition, a
+    dropped level-0 order filter, a dropped backward neighbour, a wrong
+    multiplicity) make the count differ from the brute force, which the
+    correct plan matches, or (for count-neutral ones) fail the hand-derived
+    plan vectors."""
+    import copy
+    G = T.CSRGraph.erdos_renyi(30, 150, 77)
+    for i in (1, 2, 5, 7):
+        good = O.plan(fixture(i), 0)
+        want = O.bruteforce(G.rowptr, G.colidx, None, good)
+        assert O.count(G.rowptr, G.colidx, None, good)["count"] == want
+        k = int(good.k)
+        planted = []
+        # 1. a GT condition of a DFS level (u >= 2) turned into NE (a
+        #    symmetry pair lost); vertex 1's is the level-0 filter (sym01)
+        for u in range(2, k):
+            for j in range(u):
+                if good.cond_op[u][j] == 1:
+                    p = copy.deepcopy(good)
+                    p.cond_op[u][j] = 2
+                    planted.append(("dropped condition", p))
+                    break
+            else:
+                continue
+            break
+        # 2. a GT condition flipped to LT
+        for u in range(2, k):
+            for j in range(u):
+                if good.cond_op[u][j] == 1:
+                    p = copy.deepcopy(good)
+                    p.cond_op[u][j] = 0
+                    planted.append(("flipped condition", p))
+                    break
+            else:
+                continue
+            break
+        # 3. the last backward neighbour of the last vertex dropped
+        u = k - 1
+        if good.num_bn[u] >= 2:
+            p = copy.deepcopy(good)
+            p.num_bn[u] -= 1
+            p.shared_lvl[u] = -1
+            p.num_bn_sh[u] = 0
+            planted.append(("dropped backward neighbour", p))
+        # 3b. the level-0 order filter (r < c) dropped
+        if good.sym01:
+            p = copy.deepcopy(good)
+            p.sym01 = 0
+            planted.append(("dropped level-0 order", p))
+        # 4. multiplicity off by a factor
+        p = copy.deepcopy(good)
+        p.mult = good.mult * 2
+        planted.append(("wrong multiplicity", p))
+        assert planted
+
+        def vectors_ok(p):  # the hand-derived plan vectors of this fixture
+            shape = (i - 1) % 7 + 1
+            return p.mult == AUT[shape] and p.sym01 == int((0, 1) in PAIRS[shape]) and all(
+                p.cond_op[u][j] == (1 if (j, u) in PAIRS[shape] else 2) for u in range(k) for j in range(u))
+
+        assert vectors_ok(good)
+        for what, p in planted:
+            got = O.count(G.rowptr, G.colidx, None, p, reuse=False)["count"]
+            # caught by the count (brute force) or, for count-neutral mistakes
+            # such as flipping the order between two interchangeable
+            # vertices, by the plan vectors
+            assert got != want or not vectors_ok(p), (i, what)
