@@ -116,6 +116,26 @@ def test_random_patterns(dev):
             check(dev, dg, G, text, labeled)
 
 
+def test_sparse_random_patterns(dev):
+    """Sparse seeded patterns (trees and near-trees, k 4-6), whose hoisted
+    intersections mostly do not involve the previous vertex; default and
+    forced hoisting, with donation."""
+    G = T.CSRGraph.chung_lu(1500, 6000, 2.1, 60, 23).with_labels(3, 6)
+    dg = dev.upload(G)
+    base_t = dev.tuning()
+    rng = np.random.default_rng(24)
+    for j in range(20):
+        k = int(rng.integers(4, 7))
+        m = int(rng.integers(k - 1, k + 2))
+        text = T.gen_pattern(k, m, 3, int(rng.integers(1 << 62)))
+        for labeled in (0, 1):
+            mode = dict(hoist_min=1, timeout_us=1, donate_levels=5) if j % 2 else dict()
+            say("sparse", j, k, m, labeled, mode)
+            check(dev, dg, G, text, labeled, **mode)
+            dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_ if f != "reserved"})
+    dg.free()
+
+
 def test_scheduling_modes_are_exact(dev):
     """Forced scheduler paths: donate at every check, a tiny task ring (full
     ring path), odd chunk sizes, reuse off; counts and algorithmic bytes are
