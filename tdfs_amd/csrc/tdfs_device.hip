@@ -160,10 +160,10 @@ struct KParams {
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
   int hoist_min;             // build H(u) for a level of at least this many candidates (0: never)
-  // enumeration (null emit: count only): embeddings as rows of k ids
-  int *emit;
+  // enumeration (emit_n non-null): embeddings as rows of k ids
+  int *emit;                    // rows buffer (null when emit_cap is 0)
   unsigned long long emit_cap;  // rows
-  unsigned long long *emit_n;   // rows produced (also past emit_cap)
+  unsigned long long *emit_n;   // rows produced (also past emit_cap); null: count only
 };
 
 struct WaveLds {
@@ -402,7 +402,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
   unsigned n = 0;
   if (start < stop) {
-    if (leaf && nl == 1 && label_known && !P.emit) {
+    if (leaf && nl == 1 && label_known && !P.emit_n) {
       // Count-only leaf over one list: |list ∩ (lo,hi)| minus the excluded
       // path vertices that lie in that window and in the list.
       n = (unsigned)(stop - start);
@@ -440,7 +440,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         if (out) {
           const unsigned pos = n + mbcnt64(mask);
           if (ok) out[pos] = x;
-        } else if (leaf && P.emit) {
+        } else if (leaf && P.emit_n) {
           emit_rows(P, S, u, x, ok, mask);
         }
         n += (unsigned)__popcll(mask);
@@ -2316,7 +2316,7 @@ extern "C" int tdfs_enumerate(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_pla
     release();
     return fail(TDFS_E_OOM, "enumeration counter");
   }
-  P.emit = d_rows ? d_rows : reinterpret_cast<int *>(d_n);  // non-null marks enumeration
+  P.emit = d_rows;
   P.emit_cap = cap;
   P.emit_n = d_n;
   P.leaf_batch = 0;  // leaves are walked one parent at a time to write their rows
