@@ -577,13 +577,14 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < nb; i++) add_adj(pl->bn[u][i]);
   }
   const int nl = nu + (hasA ? 1 : 0) + (hasB ? 1 : 0);
-  if (L > 0 && nu == 1 && hasA && !hasB && S.uln[0] <= kWave) {
-    // One shared input U (C(dep), H(u) or N(M(b)), b <= L) of at most 64
+  if (L > 0 && nu == 1 && hasA && !hasB && S.uln[0] <= 2 * kWave) {
+    // One shared input U (C(dep), H(u) or N(M(b)), b <= L) of at most 128
     // elements and the owner's N(M(L+1)).  U is the probe for every owner
     // (set semantics: the probe does not change C(u)): lane j holds U[j mod
-    // g], filtered once against everything but the owner's vertex, and 64 / g
-    // owners are tested per step (g = |U| rounded up to a power of two) -- no
-    // pair scan, no owner search.
+    // g] (and U[64 + j] when |U| > 64), filtered once against everything but
+    // the owner's vertex, and 64 / g owners are tested per step (g = |U|
+    // rounded up to a power of two, at most 64) -- no pair scan, no owner
+    // search.
     const int *U = S.ulp[0];
     const int nU = S.uln[0];
     taken = i1 - i0;
@@ -598,21 +599,28 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     const bool c_gt = !labeled && ((gt >> ju) & 1);
     const bool c_lt = !labeled && ((lt >> ju) & 1);
     const bool c_ne = !c_gt && !c_lt && ((same >> ju) & 1);
+    auto shared_ok = [&](int y) -> bool {  // conditions against M(0..u-2) and the label
+      bool ok = true;
+#pragma unroll 1
+      for (int i = 0; i < ju; i++) {
+        const int m = S.M[i];
+        if (!labeled && ((gt >> i) & 1)) ok = ok && y > m;
+        else if (!labeled && ((lt >> i) & 1)) ok = ok && y < m;
+        else if ((same >> i) & 1) ok = ok && y != m;
+      }
+      if (labeled && seg < 0 && ok) ok = (int)P.labels[y] == pl->label[u];
+      return ok;
+    };
     int g = 1;
-    while (g < nU) g <<= 1;
+    while (g < nU && g < kWave) g <<= 1;
     const int per = kWave / g;
     const int j = lane & (g - 1);
     const int slot = lane / g;
     const int y = j < nU ? U[j] : 0;
-    bool yv = j < nU;
-#pragma unroll 1
-    for (int i = 0; i < ju; i++) {
-      const int m = S.M[i];
-      if (!labeled && ((gt >> i) & 1)) yv = yv && y > m;
-      else if (!labeled && ((lt >> i) & 1)) yv = yv && y < m;
-      else if ((same >> i) & 1) yv = yv && y != m;
-    }
-    if (labeled && seg < 0 && yv) yv = (int)P.labels[y] == pl->label[u];
+    const bool yv = j < nU && shared_ok(y);
+    const bool two = nU > kWave;  // then g = 64: one owner per step, two elements per lane
+    const int y2 = two && kWave + lane < nU ? U[kWave + lane] : 0;
+    const bool yv2 = two && kWave + lane < nU && shared_ok(y2);
     const int owners = i1 - i0;
     unsigned long long found = 0;
     for (int o0 = 0; o0 < owners; o0 += per) {
@@ -623,12 +631,16 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
       const int h = __shfl(vh0, oo);
       const int *lp = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(vp0), oo));
       const int ln = __shfl(vn0, oo);
-      bool ok = yv && ov;
-      if (c_gt) ok = ok && y > x;
-      if (c_lt) ok = ok && y < x;
-      if (c_ne) ok = ok && y != x;
-      if (ok) ok = h >= 0 ? bit_test(P.hub_bits + (long long)h * P.hub_words, y) : contains(lp, ln, y);
-      found += (unsigned)__popcll(__ballot(ok));
+      auto test = [&](int yy, bool v) -> bool {
+        bool ok = v && ov;
+        if (c_gt) ok = ok && yy > x;
+        if (c_lt) ok = ok && yy < x;
+        if (c_ne) ok = ok && yy != x;
+        if (ok) ok = h >= 0 ? bit_test(P.hub_bits + (long long)h * P.hub_words, yy) : contains(lp, ln, yy);
+        return ok;
+      };
+      found += (unsigned)__popcll(__ballot(test(y, yv)));
+      if (two) found += (unsigned)__popcll(__ballot(test(y2, yv2)));
     }
     return found;
   }
