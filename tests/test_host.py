@@ -240,3 +240,42 @@ def test_query_stream_rule_terminates_and_is_seeded():
     b = serve.query_stream(HostEstimator(), 12, 5, 0.3, 8, tri / 4)
     assert [q["text"] for q in a] == [q["text"] for q in b]
     assert all(q["est_cost"] <= tri / 4 for q in a)
+
+
+def test_tuning_knobs_are_abi_fields():
+    """Every knob the autotuner sweeps (and bench applies from its results)
+    is a field of tdfs_tuning, and every tuned point on file names only such
+    fields; the kernel-variant names bench reports cover every register budget."""
+    import json
+    import bench
+    from tdfs_amd import autotune
+    fields = {f for f, _ in _lib.Tuning._fields_}
+    assert set(autotune.SPACE) <= fields and set(autotune.DEFAULT) <= fields
+    tuned = json.loads((ROOT / "profiles" / "autotune.json").read_text())
+    for w, r in tuned.items():
+        assert set(r["best"]) <= fields, w
+    assert set(autotune.SPACE["waves_per_simd"]) <= set(bench.KERNEL_NAMES)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of tdfs_plan, tdfs_tuning and tdfs_result have the
+    C header's sizes and field offsets (compiled here with gcc)."""
+    import ctypes
+    import subprocess
+    structs = {"tdfs_plan": _lib.Plan, "tdfs_tuning": _lib.Tuning, "tdfs_result": _lib.Result}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tdfs.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
