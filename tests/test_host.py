@@ -279,3 +279,22 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_work_estimate_is_unbiased_and_seeded():
+    """tdfs_estimate_work samples level-0 edges (with replacement, in 16
+    edge-index strata, each scaled by its size): the same seed gives the same
+    estimate, and the mean raw estimate over many seeds is the exact raw
+    count (oracle) within a few percent."""
+    from oracle import oracle as O
+    G = T.CSRGraph.chung_lu(3000, 15000, 2.1, 200, 44).with_labels(2, 5)
+    for i, labeled in ((1, 0), (2, 0), (1, 1), (4, 1)):
+        text = (GOLDEN / "patterns" / f"{i}.g").read_text()
+        plan = T.plan_pattern(text, labeled)
+        exact = O.count(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(text, labeled), threads=4)["raw"]
+        a = T.estimate_work(G, plan, 2000, 7, 1e12)
+        b = T.estimate_work(G, plan, 2000, 7, 1e12)
+        assert a == b
+        raws = [T.estimate_work(G, plan, 2000, s, 1e12)["raw"] for s in range(1, 121)]
+        mean = sum(raws) / len(raws)
+        assert abs(mean - exact) <= 0.06 * exact, (i, labeled, mean, exact)
