@@ -165,6 +165,29 @@ def measure(dev, dg, pats, steps: int, warmup: int, plain_too: bool = True) -> d
     }
 
 
+def per_pattern(dev, dg, pats) -> list[dict]:
+    """One launch per pattern (tdfs_count, the tuned kernel) and one statistics
+    launch per pattern: kernel ms, count and the cycle split of each pattern,
+    slowest first."""
+    import tdfs_amd as T
+
+    rows = []
+    for name, (text, lab) in pats.items():
+        plan = T.plan_pattern(text, lab)
+        dev.set_stats(False)
+        r = dev.count(dg, plan)
+        dev.set_stats(True)
+        dev.count(dg, plan)
+        prof = dev.last_profile()
+        dev.set_stats(False)
+        tot = max(1, prof["total"])
+        rows.append({"pattern": name, "k": int(plan.k), "labeled": bool(plan.labeled), "kernel_ms": r.kernel_ms,
+                     "count": int(r.count),
+                     "cycles_frac": {k: round(prof[k] / tot, 3) for k in ("filter", "extend", "leaf", "idle")}})
+    rows.sort(key=lambda x: -x["kernel_ms"])
+    return rows
+
+
 def nan_none(x: float):
     return None if x != x else x
 
@@ -177,6 +200,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c4")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--per-pattern", action="store_true",
+                    help="instead of the bench line: each pattern alone (ms, count, cycle split), one JSON line each")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one-launch-per-pattern form (profiled runs: the match kernel then runs "
                          "only as the batched launch)")
@@ -215,6 +240,12 @@ def main() -> int:
     if tuned:
         dev.set_tuning(**tuned)
     dg = dev.upload(g)
+    if args.per_pattern:
+        for row in per_pattern(dev, dg, pats):
+            print(json.dumps(row), flush=True)
+        dg.free()
+        dev.close()
+        return 0
     m = measure(dev, dg, pats, args.steps, args.warmup, plain_too=not args.no_plain)
     dg.free()
 
