@@ -97,6 +97,9 @@ typedef struct tdfs_tuning {
   int32_t hoist_min;         /* the inputs of C(u) that do not involve M(u-1) are intersected once per
                                 M(u-2) when the level holds at least this many candidates (default 2;
                                 < 0: never) */
+  int32_t idle_deadline_ms;  /* a wave that has waited this long for work while no busy wave reported
+                                progress ends the launch with a scheduler error (TDFS_E_HIP) instead of
+                                waiting on (default 60000; < 0: never) */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */
@@ -111,7 +114,8 @@ typedef struct tdfs_result {
   uint32_t reserved;
 } tdfs_result;
 
-#define TDFS_INCOMPLETE 1u
+#define TDFS_INCOMPLETE 1u /* the time limit dropped part of this plan's work: count is a lower bound */
+#define TDFS_FAILED 2u     /* serving: the launch failed on the device, or the count overflowed 64 bits */
 
 typedef struct tdfs_ctx tdfs_ctx;
 typedef struct tdfs_graph tdfs_graph;
@@ -189,7 +193,11 @@ int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8);
 int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4);
 
 /* Copies a validated CSR to HBM (kept resident until tdfs_graph_free).
- * labels: int32[n] or NULL (unlabeled). */
+ * labels: int32[n] or NULL (unlabeled).  Any int32 label values are kept
+ * (src/graph.h:153-162 reads 32-bit labels): the distinct values are mapped
+ * densely on upload (at most 65,535 of them, else TDFS_E_ARG) and pattern
+ * labels go through the same map; a pattern label that no vertex carries
+ * matches nothing. */
 int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const int64_t *rowptr,
                       const int32_t *colidx, const int32_t *labels, tdfs_graph **out);
 void tdfs_graph_free(tdfs_graph *g);
@@ -244,20 +252,36 @@ void tdfs_batch_destroy(tdfs_batch *b);
 /* ---------------- serving (continuous batching) ---------------- */
 typedef struct tdfs_server tdfs_server;
 /* `slots` concurrent launches over the resident graph, each on its own
- * stream with its own workspace and a persistent grid of blocks_per_cu
- * workgroups per CU; up to max_batch waiting queries start together on an idle
- * slot.  budget > 0: queries whose sampled work estimate exceeds it are
- * refused with TDFS_E_BUDGET.  Device memory is fixed at creation. */
+ * stream with its own task ring and a persistent grid of blocks_per_cu
+ * workgroups per CU.  Admission is by DFS-stack memory: the server holds an
+ * arena of stack_mb MiB (0: enough for every slot to run 7-vertex plans), a
+ * launch over plans of at most kmax vertices takes waves x (2,048 + 2 (kmax-3)
+ * x dmax) x 4 bytes of it, and waiting queries (a FIFO prefix of at most
+ * max_batch) start on an idle slot as soon as such a region is free.  A
+ * query whose stacks can never fit is refused with TDFS_E_BUDGET; so is one
+ * whose sampled work estimate exceeds budget, when budget > 0.  Device
+ * memory is fixed at creation (no allocation while serving).  No counterpart
+ * in the reference (one pattern per process, cu_test.cu:36-49 sizes the
+ * per-warp stacks). */
 int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slots, int32_t blocks_per_cu,
-                      int32_t max_batch, double budget, tdfs_server **out);
+                      int32_t max_batch, int64_t stack_mb, double budget, tdfs_server **out);
 /* Queues one query; returns its ticket. */
 int tdfs_serve_submit(tdfs_server *sv, const tdfs_plan *plan, uint64_t *ticket);
+/* Same, counting only level-0 directed edges [edge_begin, edge_end) (end < 0:
+ * all) in the numbering tdfs_count would use for this plan alone (as
+ * tdfs_count's range). */
+int tdfs_serve_submit_range(tdfs_server *sv, const tdfs_plan *plan, int64_t edge_begin, int64_t edge_end,
+                            uint64_t *ticket);
 /* Non-blocking progress: reports finished queries (ticket, result, latency
  * from submit to collection in seconds; at most cap) and starts idle slots on
- * waiting queries.  Call repeatedly. */
+ * waiting queries.  Each query is reported as soon as its own work has ended,
+ * while the rest of its launch may still run (result.kernel_ms is then the
+ * host time since its launch started).  Call repeatedly.  A launch that fails
+ * on the device reports its queries with TDFS_FAILED; an error return means a
+ * launch could not be started and its queries are still queued. */
 int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *results, double *latency_s, int32_t cap,
                     int32_t *nout);
-/* out4 = {queries waiting, slots busy, slots, queries running} */
+/* out4 = {queries waiting, slots busy, slots, queries running and not yet reported} */
 int tdfs_serve_status(const tdfs_server *sv, int64_t *out4);
 void tdfs_serve_destroy(tdfs_server *sv);
 

@@ -104,11 +104,12 @@ class CSRGraph:
         deg = np.diff(self.rowptr)
         rp = np.zeros(n + 1, dtype=np.int64)
         rp[1:] = np.cumsum(deg[inv])
-        src = perm[np.repeat(np.arange(n), deg)]
-        dst = perm[self.colidx]
-        order = np.lexsort((dst, src))
+        # one sort of (new row, new column) keys packed in 64 bits
+        key = perm[np.repeat(np.arange(n), deg)] << 32
+        key |= perm[self.colidx]
+        key.sort(kind="stable")
         labels = None if self.labels is None else np.ascontiguousarray(self.labels[inv]).astype(np.int32)
-        return CSRGraph(rp, dst[order].astype(np.int32), labels)
+        return CSRGraph(rp, (key & 0xFFFFFFFF).astype(np.int32), labels)
 
     def with_labels(self, nlabels: int, seed: int) -> "CSRGraph":
         labels = np.zeros(max(self.n, 1), dtype=np.int32)

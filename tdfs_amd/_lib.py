@@ -53,6 +53,7 @@ class Tuning(C.Structure):
         ("cursor_order", C.c_int32),
         ("waves_per_simd", C.c_int32),
         ("hoist_min", C.c_int32),
+        ("idle_deadline_ms", C.c_int32),
     ]
 
 
@@ -113,8 +114,10 @@ _SIGS = {
     "tdfs_batch_wait": (C.c_int, [_P, C.POINTER(Result)]),
     "tdfs_batch_run": (C.c_int, [_P, C.POINTER(Result)]),
     "tdfs_batch_destroy": (None, [_P]),
-    "tdfs_serve_create": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.POINTER(_P)]),
+    "tdfs_serve_create": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_double,
+                                    C.POINTER(_P)]),
     "tdfs_serve_submit": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(C.c_uint64)]),
+    "tdfs_serve_submit_range": (C.c_int, [_P, C.POINTER(Plan), C.c_int64, C.c_int64, C.POINTER(C.c_uint64)]),
     "tdfs_serve_poll": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(Result), C.POINTER(C.c_double), C.c_int32,
                                   C.POINTER(C.c_int32)]),
     "tdfs_serve_status": (C.c_int, [_P, _I64P]),

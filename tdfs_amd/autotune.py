@@ -119,6 +119,7 @@ def main() -> int:
     ap.add_argument("--rounds", type=int, default=1)
     ap.add_argument("--knobs", default="", help="comma-separated subset of knobs to sweep (default all)")
     ap.add_argument("--resume", action="store_true", help="start from the best point already in --out")
+    ap.add_argument("--trials-out", default="", help="also write every sweep trial (knobs, ms) to this file")
     args = ap.parse_args()
     root = Path(__file__).resolve().parent.parent
     sys.path.insert(0, str(root))
@@ -137,8 +138,10 @@ def main() -> int:
                start=start, knobs=knobs)
     res["workload"] = desc
     res["seconds"] = round(time.time() - t0, 1)
-    if args.resume and args.workload in data:
-        res["trials"] = data[args.workload].get("trials", []) + res["trials"]
+    # the record keeps the winner; the sweep log goes next to it only on request
+    trials = res.pop("trials")
+    if args.trials_out:
+        Path(args.trials_out).write_text(json.dumps(trials, indent=1))
     data[args.workload] = res
     out.write_text(json.dumps(data, indent=1))
     print(json.dumps({"workload": args.workload, "best": res["best"], "best_ms": res["best_ms"]}))

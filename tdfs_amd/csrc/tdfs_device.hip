@@ -57,7 +57,7 @@ constexpr int kMaxSegLabels = 64;          // label segments are built for at mo
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
 struct DevPlan {
-  int k, labeled, sym01, unsat;
+  int k, labeled, sym01, pad_;
   int label[8];
   int deg[8];
   int num_bn[8];
@@ -79,6 +79,9 @@ struct DevPlan {
   int hfix[2][8];
   int hx_n[2][8];
   int hx[2][8][8];
+  // level-0 directed edges of this plan: [e_lo, e_lo + e_span) (per-plan
+  // cursors; the interleaved cursor uses the launch's common range)
+  long long e_lo, e_span;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -112,6 +115,8 @@ struct Ctrl {
   int pad6[31];
   int error;
   int pad7[31];
+  unsigned long long progress;  // bumped by busy waves while they advance (idle deadline)
+  long long pad8[15];
 };
 
 struct KParams {
@@ -121,7 +126,7 @@ struct KParams {
   const int *degv;
   const int *src;
   const int *maxnbr;
-  const unsigned char *labels;
+  const unsigned short *labels;  // dense label ids (the upload maps the caller's values)
   const int *colidx_lab;      // each list re-sorted by (label, id); null when unlabeled
   const int *loff;            // per vertex: nlab+1 offsets of its label segments
   int nlab;
@@ -136,9 +141,10 @@ struct KParams {
   unsigned long long span;      // edges per plan: edge_hi - edge_lo
   unsigned long long *cursors;  // per plan (128-byte stride): next level-0 offset
   int interleave;               // 1: one cursor over (chunk, plan); 0: per-plan cursors
-  // per-wave workspace
+  // per-wave workspace: level-0 pairs, then C(2..k-2), then H(3..k-1) at hoff
   int *wbuf;
   long long wstride;
+  long long hoff;
   int dstride;
   // task ring
   TaskRec *q_data;
@@ -151,7 +157,8 @@ struct KParams {
   Ctrl *ctrl;
   // tuning
   long long timeout_ticks;  // < 0: never donate
-  long long deadline_ticks; // a wave idle this long while others stay busy gives up (error)
+  long long deadline_ticks; // a wave idle this long with no progress reported gives up (error); <= 0: never
+  long long prog_ticks;     // busy waves report progress at most this often
   int split_chunk;
   int donate_levels;
   int reuse;
@@ -160,6 +167,12 @@ struct KParams {
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
   int hoist_min;             // build H(u) for a level of at least this many candidates (0: never)
+  // per-plan completion (serving): pend[pid * 16] counts the plan's level-0
+  // chunks and ring tasks in flight; when it drops to zero with the plan's
+  // cursor spent, the plan's count and flags go to done_host[pid * 2 + {0, 1}]
+  // (host-mapped memory) so the host can collect it before the launch ends
+  unsigned long long *pend;
+  unsigned long long *done_host;
   // enumeration (emit_n non-null): embeddings as rows of k ids
   int *emit;                    // rows buffer (null when emit_cap is 0)
   unsigned long long emit_cap;  // rows
@@ -930,6 +943,8 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
   }
   const unsigned long long pos0 = ring_reserve(P, ntasks);
   if (pos0 == ~0ull) return 0;
+  // the tasks are in flight before any of them can be taken
+  if (P.pend && lane_id() == 0) fadd(&P.pend[(long long)pid * 16], (unsigned long long)ntasks);
   bool okall = true;
   for (int j = lane_id(); j < ntasks; j += kWave) {
     TaskRec t;
@@ -1010,6 +1025,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   const int k = pl->k;
   int level = floor;
   long long t0 = wall_clock64();
+  long long tprog = t0;
   const bool may_donate = P.timeout_ticks >= 0;
   for (;;) {
     const int it = uni(S.it[level]);
@@ -1023,9 +1039,13 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       if (lane_id() == 0) atomicOr(&P.stats[pid * 4 + 3], 1ull);  // this plan's count is incomplete
       return;
     }
-    if (may_donate && level < P.donate_levels) {
+    if (level < P.donate_levels) {
       const long long now = wall_clock64();
-      if (now - t0 > P.timeout_ticks) {
+      if (now - tprog > P.prog_ticks) {  // tell idle waves that this one still advances
+        tprog = now;
+        if (lane_id() == 0) fadd(&P.ctrl->progress, 1ull);
+      }
+      if (may_donate && now - t0 > P.timeout_ticks) {
         t0 = now;
         // donate only while idle waves outnumber the tasks already queued
         int idle = 0;
@@ -1080,6 +1100,23 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   }
 }
 
+// Serving: one level-0 chunk or ring task of plan pid has ended, or a chunk
+// grab found the plan's cursor spent.  The call that brings the plan's
+// in-flight count to zero while its cursor is spent publishes the plan's
+// count and flags to host-mapped memory (every chunk handed out and every
+// task donated has then ended; a second publication writes the same values).
+// Lane 0 only; the caller has waited for its own count update to complete.
+__device__ __forceinline__ void unit_done(const KParams &P, int pid) {
+  const unsigned long long old = fadd(&P.pend[(long long)pid * 16], ~0ull);
+  if (old != 1ull) return;
+  const DevPlan *pl = P.plans + pid;
+  if (ld_rlx(&P.cursors[(long long)pid * 16]) < (unsigned long long)pl->e_span) return;
+  const unsigned long long c = ld_rlx(&P.counts[pid]);
+  const unsigned long long f = ld_rlx(&P.stats[(long long)pid * 4 + 3]);
+  __hip_atomic_store(&P.done_host[pid * 2 + 0], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&P.done_host[pid * 2 + 1], 1ull | (f << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // The persistent match loop of one wave (body of the match kernels below).
 template <bool STATS>
 __device__ __forceinline__ void match_body(const KParams &P) {
@@ -1090,8 +1127,8 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   int *wb = P.wbuf + gw * P.wstride;
   int *L0r = wb;
   int *L0c = wb + kChunkMax;
-  int *lbase = wb + 2 * kChunkMax;                          // C(2..k-2): stack levels 1..k-3
-  int *hbase = lbase + (long long)(kMaxK - 3) * P.dstride;  // H(3..k-1)
+  int *lbase = wb + 2 * kChunkMax;  // C(2..k-2): stack levels 1..k-3
+  int *hbase = wb + P.hoff;         // H(3..k-1)
   WaveLds &S = lds[wib];
   clear_hoists(S);
   Ctrl *C = P.ctrl;
@@ -1132,16 +1169,26 @@ __device__ __forceinline__ void match_body(const KParams &P) {
       // per-plan cursors: this wave stays on its plan until that plan's
       // edges are handed out, then moves to the next plan with edges left
       while (!P.interleave && plans_left > 0) {
-        if (lane == 0) pos = fadd(&P.cursors[(long long)cur_plan * 16], (unsigned long long)P.chunk);
+        if (lane == 0) {
+          if (P.pend) fadd(&P.pend[(long long)cur_plan * 16], 1ull);  // a chunk may be in flight
+          pos = fadd(&P.cursors[(long long)cur_plan * 16], (unsigned long long)P.chunk);
+        }
         pos = uni64(pos);
-        if (pos < P.span) break;
+        if (pos < (unsigned long long)P.plans[cur_plan].e_span) break;
+        if (P.pend && lane == 0) unit_done(P, cur_plan);
         cur_plan = cur_plan + 1 == P.nplans ? 0 : cur_plan + 1;
         plans_left--;
       }
       if (plans_left > 0) {
         pid = cur_plan;
-        e0 = P.edge_lo + (long long)pos;
-        e1 = min(e0 + (long long)P.chunk, P.edge_hi);
+        if (P.interleave) {
+          e0 = P.edge_lo + (long long)pos;
+          e1 = min(e0 + (long long)P.chunk, P.edge_hi);
+        } else {
+          const DevPlan *cp = P.plans + pid;
+          e0 = cp->e_lo + (long long)pos;
+          e1 = min(e0 + (long long)P.chunk, cp->e_lo + cp->e_span);
+        }
         kind = 1;
       } else {
         if (lane == 0) fadd(&C->busy, -1);
@@ -1149,18 +1196,29 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           if (lane == 0) fadd(&C->idle, 1);
           is_idle = true;
         }
-        const long long t_idle = wall_clock64();
+        long long t_idle = wall_clock64();
+        unsigned long long seen = lane == 0 ? ld_rlx(&C->progress) : 0ull;
         const long long tpi = STATS ? clock64() : 0;
         int backoff = 1;
         for (;;) {
           int stop = 0;
           if (lane == 0) {
-            if (ld_rlx(&C->error)) stop = 1;
-            else if (ld_rlx(&C->q_avail) <= 0 && ld_rlx(&C->busy) == 0 && ld_rlx(&C->q_avail) <= 0)
+            if (ld_rlx(&C->error)) {
               stop = 1;
-            else if (wall_clock64() - t_idle > P.deadline_ticks) {
-              __hip_atomic_store(&C->error, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (ld_rlx(&C->q_avail) <= 0 && ld_rlx(&C->busy) == 0 && ld_rlx(&C->q_avail) <= 0) {
               stop = 1;
+            } else if (P.deadline_ticks > 0) {
+              // give up only when no busy wave has reported progress for the
+              // whole deadline: a lost handshake, never a long subtree
+              const long long now = wall_clock64();
+              const unsigned long long pr = ld_rlx(&C->progress);
+              if (pr != seen) {
+                seen = pr;
+                t_idle = now;
+              } else if (now - t_idle > P.deadline_ticks) {
+                __hip_atomic_store(&C->error, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                stop = 1;
+              }
             }
           }
           if (uni(stop)) break;
@@ -1199,7 +1257,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
     const bool expired = P.limit_ticks > 0 && wall_clock64() - t_wave > P.limit_ticks;
     if (expired) {
       if (lane == 0) atomicOr(&P.stats[pid * 4 + 3], 1ull);  // dropped: count incomplete
-    } else if (!pl->unsat) {
+    } else {
       int floor = -1;  // stack level the DFS starts from (one call site for both kinds of work)
       if (kind == 1) {
         const long long tp = STATS ? clock64() : 0;
@@ -1243,10 +1301,18 @@ __device__ __forceinline__ void match_body(const KParams &P) {
       if (floor >= 0) dfs<STATS>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
     }
     if (lane == 0) {
-      if (cnt) atomicAdd(&P.counts[pid], cnt);
+      if (cnt) {
+        const unsigned long long old = fadd(&P.counts[pid], cnt);
+        if (old + cnt < old) atomicOr(&P.stats[pid * 4 + 3], 2ull);  // the raw count wrapped past 2^64
+      }
       if (don) {
         atomicAdd(&P.stats[pid * 4 + 2], don);
         don = 0;
+      }
+      if (P.pend) {
+        // this unit's count is in place before the unit stops counting as in flight
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        unit_done(P, pid);
       }
       fadd(&C->busy, -1);
     }
@@ -1319,7 +1385,7 @@ struct GraphView {
   int *degv = nullptr;
   int *src = nullptr;
   int *maxnbr = nullptr;
-  unsigned char *labels = nullptr;
+  unsigned short *labels = nullptr;  // dense label ids
   int *hub_idx = nullptr;
   unsigned *hub_bits = nullptr;
   long long hub_words = 0;
@@ -1335,6 +1401,9 @@ struct tdfs_graph {
   int64_t nnz;
   int32_t dmax;
   int labeled;
+  // distinct label values of the caller, ascending: dense id i stands for
+  // label_values[i] on the device
+  std::vector<int32_t> label_values;
   // view[0]: the numbering as given.  view[1] (when `ordered`): vertices
   // renumbered by ascending degree (ties by id), so that the symmetry-breaking
   // "smaller id first" conditions start the DFS at low-degree vertices.
@@ -1357,6 +1426,7 @@ struct Launch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int *wbuf = nullptr;
   size_t wbuf_ints = 0;
+  bool wbuf_external = false;  // wbuf is a region of memory the launch does not own (serving arena)
   TaskRec *q_data = nullptr;
   unsigned long long *q_seq = nullptr;
   unsigned long long q_cap = 0;
@@ -1411,6 +1481,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->waves_per_simd = 0;
   t->leaf_pair_cap = 4096;
   t->hoist_min = 2;
+  t->idle_deadline_ms = 60000;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1436,6 +1507,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->leaf_pair_cap <= 0) t->leaf_pair_cap = d.leaf_pair_cap;
   if (t->leaf_pair_cap < 64) t->leaf_pair_cap = 64;
   if (t->hoist_min == 0) t->hoist_min = d.hoist_min;
+  if (t->idle_deadline_ms == 0) t->idle_deadline_ms = d.idle_deadline_ms;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -1454,7 +1526,7 @@ static void launch_free(Launch &L) {
   if (L.stream) (void)hipStreamSynchronize(L.stream);
   if (L.exec) (void)hipGraphExecDestroy(L.exec);
   if (L.graph) (void)hipGraphDestroy(L.graph);
-  if (L.wbuf) (void)hipFree(L.wbuf);
+  if (L.wbuf && !L.wbuf_external) (void)hipFree(L.wbuf);
   if (L.q_data) (void)hipFree(L.q_data);
   if (L.q_seq) (void)hipFree(L.q_seq);
   if (L.ctrl) (void)hipFree(L.ctrl);
@@ -1562,10 +1634,11 @@ static void free_view(GraphView &v) {
 }
 
 // Copies one numbering of a validated CSR to the device with its derived
-// arrays (degree, row of each edge, max neighbour degree, uint8 labels) and
-// the dense neighbour bitmaps of its highest-degree vertices.
+// arrays (degree, row of each edge, max neighbour degree, dense 16-bit label
+// ids) and the dense neighbour bitmaps of its highest-degree vertices.
+// labels: dense ids in [0, nlabels) or null.
 static int upload_view(const tdfs_tuning &t, int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx,
-                       const int32_t *labels, GraphView &g) {
+                       const int32_t *labels, int nlabels, GraphView &g) {
   std::vector<int> degv(n > 0 ? n : 1), src(nnz > 0 ? nnz : 1), maxnbr(n > 0 ? n : 1);
   for (int32_t v = 0; v < n; v++) degv[v] = (int)(rowptr[v + 1] - rowptr[v]);
   for (int32_t v = 0; v < n; v++) {
@@ -1586,10 +1659,10 @@ static int upload_view(const tdfs_tuning &t, int32_t n, int64_t nnz, const int64
   if ((e = hipMalloc(&g.degv, sizeof(int) * (size_t)std::max(n, 1)))) return bail("degv", e);
   if ((e = hipMalloc(&g.src, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("src", e);
   if ((e = hipMalloc(&g.maxnbr, sizeof(int) * (size_t)std::max(n, 1)))) return bail("maxnbr", e);
-  if ((e = hipMalloc(&g.labels, (size_t)std::max(n, 1)))) return bail("labels", e);
-  std::vector<unsigned char> lab8(n > 0 ? n : 1, 255);
+  if ((e = hipMalloc(&g.labels, sizeof(unsigned short) * (size_t)std::max(n, 1)))) return bail("labels", e);
+  std::vector<unsigned short> lab16(n > 0 ? n : 1, 0);
   if (labels)
-    for (int32_t v = 0; v < n; v++) lab8[v] = (labels[v] >= 0 && labels[v] < 255) ? (unsigned char)labels[v] : 255;
+    for (int32_t v = 0; v < n; v++) lab16[v] = (unsigned short)labels[v];
   if ((e = hipMemcpy(g.rowptr, rowptr, sizeof(long long) * (size_t)(n + 1), hipMemcpyHostToDevice)))
     return bail("copy rowptr", e);
   if (nnz > 0) {
@@ -1601,22 +1674,22 @@ static int upload_view(const tdfs_tuning &t, int32_t n, int64_t nnz, const int64
     if ((e = hipMemcpy(g.degv, degv.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice))) return bail("copy degv", e);
     if ((e = hipMemcpy(g.maxnbr, maxnbr.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice)))
       return bail("copy maxnbr", e);
-    if ((e = hipMemcpy(g.labels, lab8.data(), (size_t)n, hipMemcpyHostToDevice))) return bail("copy labels", e);
+    if ((e = hipMemcpy(g.labels, lab16.data(), sizeof(unsigned short) * (size_t)n, hipMemcpyHostToDevice)))
+      return bail("copy labels", e);
   }
   // Label segments: every list re-sorted by (label, id) with per-vertex
   // offsets, so a labeled extension reads only the vertices of its label.
   if (labels && n > 0) {
-    int nlab = 0;
-    for (int32_t v = 0; v < n; v++) nlab = std::max(nlab, (int)lab8[v] + 1);
+    const int nlab = nlabels;
     if (nlab <= kMaxSegLabels) {
       const int W = nlab + 1;
       std::vector<int> off((size_t)n * W, 0), cl(nnz > 0 ? nnz : 1);
       for (int32_t v = 0; v < n; v++) {
         int *o = off.data() + (size_t)v * W;
-        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) o[lab8[colidx[e]] + 1]++;
+        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) o[lab16[colidx[e]] + 1]++;
         for (int l = 0; l < nlab; l++) o[l + 1] += o[l];
         std::vector<int> fill(o, o + nlab);
-        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) cl[rowptr[v] + fill[lab8[colidx[e]]]++] = colidx[e];
+        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) cl[rowptr[v] + fill[lab16[colidx[e]]]++] = colidx[e];
       }
       if ((e = hipMalloc(&g.colidx_lab, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)))) return bail("colidx_lab", e);
       if ((e = hipMalloc(&g.loff, sizeof(int) * off.size()))) return bail("loff", e);
@@ -1680,7 +1753,24 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
   for (int32_t v = 0; v < n; v++) g->dmax = std::max<int32_t>(g->dmax, (int32_t)(rowptr[v + 1] - rowptr[v]));
   g->labeled = labels != nullptr;
   g->ordered = 0;
-  rc = upload_view(t, n, nnz, rowptr, colidx, labels, g->view[0]);
+  // dense label ids: the rank of each distinct value
+  std::vector<int32_t> dense;
+  if (labels) {
+    g->label_values.assign(labels, labels + n);
+    std::sort(g->label_values.begin(), g->label_values.end());
+    g->label_values.erase(std::unique(g->label_values.begin(), g->label_values.end()), g->label_values.end());
+    if (g->label_values.size() > 65535) {
+      const size_t nl = g->label_values.size();
+      delete g;
+      return fail(TDFS_E_ARG, "%zu distinct labels (at most 65535)", nl);
+    }
+    dense.resize(n);
+    for (int32_t v = 0; v < n; v++)
+      dense[v] = (int32_t)(std::lower_bound(g->label_values.begin(), g->label_values.end(), labels[v]) -
+                           g->label_values.begin());
+  }
+  const int nlabels = (int)g->label_values.size();
+  rc = upload_view(t, n, nnz, rowptr, colidx, labels ? dense.data() : nullptr, nlabels, g->view[0]);
   if (rc) {
     delete g;
     return rc;
@@ -1716,11 +1806,16 @@ extern "C" int tdfs_graph_upload(tdfs_ctx *ctx, int32_t n, int64_t nnz, const in
       for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) *dst++ = g->perm[colidx[e]];
       std::sort(ci2.data() + rp2[r], ci2.data() + rp2[r + 1]);
     }
+    std::vector<int32_t> dense2;
     if (labels) {
       lab2.resize(n);
-      for (int32_t v = 0; v < n; v++) lab2[g->perm[v]] = labels[v];
+      dense2.resize(n);
+      for (int32_t v = 0; v < n; v++) {
+        lab2[g->perm[v]] = labels[v];
+        dense2[g->perm[v]] = dense[v];
+      }
     }
-    rc = upload_view(t, n, nnz, rp2.data(), ci2.data(), labels ? lab2.data() : nullptr, g->view[1]);
+    rc = upload_view(t, n, nnz, rp2.data(), ci2.data(), labels ? dense2.data() : nullptr, nlabels, g->view[1]);
     if (rc) {
       free_view(g->view[0]);
       delete g;
@@ -1776,15 +1871,25 @@ extern "C" int tdfs_graph_estimate(const tdfs_graph *g, const tdfs_plan *plan, i
                         g->h_deg[v].data(), g->h_maxnbr[v].data());
 }
 
-static void to_dev_plan(const tdfs_plan &p, DevPlan &d) {
+// label_values: the graph's distinct labels (ascending); a labeled plan's
+// labels become their dense ids, and a label no vertex carries becomes the
+// id one past the last, which no vertex carries (the plan still runs, so its
+// statistics are those of the reference's DFS).
+static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_values, DevPlan &d) {
   std::memset(&d, 0, sizeof(d));
   d.k = p.k;
   d.labeled = p.labeled;
   d.sym01 = p.sym01;
-  d.unsat = 0;
   for (int u = 0; u < p.k; u++) {
     d.label[u] = p.label[u];
-    if (p.labeled && (p.label[u] < 0 || p.label[u] >= 255)) d.unsat = 1;  // no graph vertex carries it
+    if (p.labeled) {
+      auto it = std::lower_bound(label_values.begin(), label_values.end(), p.label[u]);
+      if (it == label_values.end() || *it != p.label[u]) {
+        d.label[u] = (int)label_values.size();  // carried by no vertex
+      } else {
+        d.label[u] = (int)(it - label_values.begin());
+      }
+    }
     d.deg[u] = p.deg[u];
     d.num_bn[u] = p.num_bn[u];
     for (int i = 0; i < p.num_bn[u]; i++) d.bn[u][i] = p.bn[u][i];
@@ -1842,7 +1947,11 @@ static int check_plan(const tdfs_plan &p) {
 
 static int launch_reserve(Launch &L, int nplans, long long waves, long long wstride, unsigned long long qcap) {
   const size_t need = (size_t)waves * (size_t)wstride;
-  if (need > L.wbuf_ints) {
+  if (L.wbuf_external) {
+    if (need > L.wbuf_ints)
+      return fail(TDFS_E_BUDGET, "workspace of %zu bytes exceeds the region of %zu bytes", need * sizeof(int),
+                  L.wbuf_ints * sizeof(int));
+  } else if (need > L.wbuf_ints) {
     if (L.wbuf) HIP_TRY(hipFree(L.wbuf));
     L.wbuf = nullptr;
     L.wbuf_ints = 0;
@@ -1884,10 +1993,24 @@ static int launch_reserve(Launch &L, int nplans, long long waves, long long wstr
   return TDFS_OK;
 }
 
+// Per-wave workspace (ints) for plans of at most kmax vertices: the level-0
+// edge pairs, C(2..kmax-2) and H(3..kmax-1), each list dstride long.
+static long long wave_ints(int kmax, int dstride) {
+  return 2ll * kChunkMax + 2ll * std::max(0, kmax - 3) * dstride;
+}
+
+static int graph_dstride(const tdfs_graph *g) { return std::max(64, (g->dmax + 63) / 64 * 64); }
+
 // Checks the plans, sizes and fills the launch's buffers and parameters.
 // blocks_cap > 0 limits the persistent grid to that many workgroups per CU.
+// ranges (2 per plan, may be null): each plan's own level-0 edge range
+// [ranges[2i], ranges[2i+1]) (end < 0: all), counted with per-plan cursors.
+// fixed: every buffer of L is already in place (plan table of plan_cap
+// entries, ring of q_cap tasks, workspace of wbuf_ints); nothing is
+// allocated, and the grid shrinks to the workgroups whose stacks fit.
 static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, const tdfs_plan *plans, int nplans,
-                          int64_t edge_begin, int64_t edge_end, bool stats, int blocks_cap, KParams &P) {
+                          int64_t edge_begin, int64_t edge_end, bool stats, int blocks_cap, KParams &P,
+                          const int64_t *ranges = nullptr, bool fixed = false) {
   if (!ctx || !g || !plans || nplans < 1) return fail(TDFS_E_ARG, "bad arguments");
   if (g->device != ctx->device) return fail(TDFS_E_ARG, "graph and context on different devices");
   for (int i = 0; i < nplans; i++) {
@@ -1903,23 +2026,44 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   if (edge_begin > edge_end) edge_begin = edge_end;
   const int chunk = t.chunk_edges;
   const long long span = edge_end - edge_begin;
-  const int dstride = std::max(64, (g->dmax + 63) / 64 * 64);
-  // level-0 pairs, C(2..5), H(3..6)
-  const long long wstride = 2ll * kChunkMax + 2ll * (kMaxK - 3) * dstride;
+  const int dstride = graph_dstride(g);
+  int kmax = 3;
+  for (int i = 0; i < nplans; i++) kmax = std::max(kmax, (int)plans[i].k);
+  const long long wstride = wave_ints(kmax, dstride);
   const int wv = stats ? 0 : t.waves_per_simd;
   const int occ = stats ? ctx->blocks_per_cu_stats
                         : (wv > 0 ? ctx->blocks_per_cu_w[wv] : ctx->blocks_per_cu);
   int bpc = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, occ) : occ;
   if (blocks_cap > 0) bpc = std::min(bpc, blocks_cap);
-  const int grid = ctx->cus * bpc;
-  const long long waves = (long long)grid * kWavesPerBlock;
-  int rc = launch_reserve(L, nplans, waves, wstride, 1ull << t.queue_log2);
-  if (rc) return rc;
+  int grid = ctx->cus * bpc;
+  if (fixed) {
+    const long long fit = (long long)(L.wbuf_ints / (size_t)wstride) / kWavesPerBlock;
+    if (fit < 1)
+      return fail(TDFS_E_BUDGET, "one workgroup's DFS stacks need %lld bytes (%d-vertex plans, max degree %d); "
+                  "the buffer holds %zu", (long long)wstride * kWavesPerBlock * (long long)sizeof(int), kmax,
+                  g->dmax, L.wbuf_ints * sizeof(int));
+    if (nplans > L.plan_cap) return fail(TDFS_E_ARG, "%d plans, room for %d", nplans, L.plan_cap);
+    grid = (int)std::min<long long>(grid, fit);
+  } else {
+    const long long waves = (long long)grid * kWavesPerBlock;
+    int rc = launch_reserve(L, nplans, waves, wstride, 1ull << t.queue_log2);
+    if (rc) return rc;
+  }
   L.grid = grid;
   bool all_sound = true;
   for (int i = 0; i < nplans; i++) all_sound = all_sound && prune_is_sound(plans[i]);
   const GraphView &V = (g->ordered && all_sound) ? g->view[1] : g->view[0];
-  for (int i = 0; i < nplans; i++) to_dev_plan(plans[i], L.h_plans[i]);
+  for (int i = 0; i < nplans; i++) {
+    to_dev_plan(plans[i], g->label_values, L.h_plans[i]);
+    long long lo = edge_begin, hi = edge_end;
+    if (ranges) {
+      lo = std::max<int64_t>(0, ranges[2 * i]);
+      hi = ranges[2 * i + 1] < 0 || ranges[2 * i + 1] > g->nnz ? g->nnz : ranges[2 * i + 1];
+      if (lo > hi) lo = hi;
+    }
+    L.h_plans[i].e_lo = lo;
+    L.h_plans[i].e_span = hi - lo;
+  }
   HIP_TRY(hipMemcpyAsync(L.plans, L.h_plans, sizeof(DevPlan) * nplans, hipMemcpyHostToDevice, L.stream));
 
   std::memset(&P, 0, sizeof(P));
@@ -1942,9 +2086,10 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.edge_hi = edge_end;
   P.span = (unsigned long long)span;
   P.cursors = L.cursors;
-  P.interleave = t.cursor_order == 1 ? 1 : 0;
+  P.interleave = (t.cursor_order == 1 && !ranges) ? 1 : 0;
   P.wbuf = L.wbuf;
   P.wstride = wstride;
+  P.hoff = 2ll * kChunkMax + (long long)std::max(0, kmax - 3) * dstride;
   P.dstride = dstride;
   P.q_data = L.q_data;
   P.q_seq = L.q_seq;
@@ -1955,7 +2100,8 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.ctrl = L.ctrl;
   const long long ticks_per_us = std::max(1, ctx->wallclock_khz / 1000);
   P.timeout_ticks = t.timeout_us < 0 ? -1 : (long long)t.timeout_us * ticks_per_us;
-  P.deadline_ticks = 300ll * 1000 * 1000 * ticks_per_us;  // 5 minutes
+  P.deadline_ticks = t.idle_deadline_ms < 0 ? 0 : (long long)t.idle_deadline_ms * 1000 * ticks_per_us;
+  P.prog_ticks = 10ll * 1000 * ticks_per_us;  // 10 ms
   P.split_chunk = t.split_chunk;
   P.donate_levels = t.donate_levels;
   P.reuse = t.reuse == 1 ? 1 : 0;
@@ -1999,8 +2145,11 @@ static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
 // Reads the pinned result mirrors after the launch's stream has drained.
 static int launch_collect(const Launch &L, const tdfs_plan *plans, int nplans, double ms, tdfs_result *out,
                           unsigned long long *prof_out) {
-  if (L.h_ctrl->error)
-    return fail(TDFS_E_HIP, "device scheduler error %d (ring handshake or deadline)", L.h_ctrl->error);
+  if (L.h_ctrl->error) {
+    const int e = L.h_ctrl->error;
+    return fail(TDFS_E_HIP, "device scheduler error %d (%s)", e,
+                e == 4 ? "idle deadline: no wave reported progress" : "task ring handshake");
+  }
   if (prof_out) std::memcpy(prof_out, L.h_prof, 8 * sizeof(unsigned long long));
   for (int i = 0; i < nplans; i++) {
     tdfs_result &r = out[i];
@@ -2011,6 +2160,7 @@ static int launch_collect(const Launch &L, const tdfs_plan *plans, int nplans, d
     r.donated = L.h_stats[4 * i + 2];
     r.flags = (uint32_t)(L.h_stats[4 * i + 3] & 1ull);
     r.kernel_ms = ms;
+    if (L.h_stats[4 * i + 3] & 2ull) return fail(TDFS_E_OVERFLOW, "raw count of plan %d overflows 64 bits", i);
     if (plans[i].labeled) {
       r.count = r.raw;
     } else {
@@ -2131,24 +2281,36 @@ extern "C" void tdfs_batch_destroy(tdfs_batch *b) {
 }
 
 // ---------------------------------------------------------------------------
-// Serving: continuous batching over a device-resident graph.  The server owns
-// `slots` launches, each with its own stream, workspace, ring and result
-// buffers, and a persistent grid capped at blocks_per_cu workgroups per CU so
-// that the slots' kernels run side by side.  Submitted queries wait in a FIFO;
-// every poll collects the slots whose launch has finished and starts each idle
-// slot on up to max_batch waiting queries.  Admission: a query whose sampled
-// work estimate exceeds the budget (or is capped) is refused; the device memory
-// of all slots is fixed at creation.
+// Serving: continuous batching over a device-resident graph.
+//
+// The server owns `slots` launches (own stream, task ring, control block,
+// plan table), a device-memory arena for the DFS stacks of the running
+// launches, and host-mapped completion records.  Submitted queries wait in a
+// FIFO.  Admission is by DFS-stack memory: a launch over plans of at most
+// kmax vertices needs waves x (level-0 pairs + 2 (kmax - 3) lists of dmax
+// entries) of stack, carved from the arena; the next waiting queries start on
+// an idle slot as soon as a region of that size is free, so light (small-k)
+// queries pack more launches into the same memory.  Every plan of a running
+// launch reports its own completion (count and flags in host-mapped memory,
+// written by the wave that ends the plan's last unit of work), so a query is
+// collected as soon as its own work is done, not when its batch-mates are.
 // ---------------------------------------------------------------------------
 struct ServeQuery {
   uint64_t ticket;
   tdfs_plan plan;
+  int64_t lo, hi;  // level-0 edge range (hi < 0: all)
+  int view;        // numbering counted on: 1 degree-ordered copy, 0 as given
   double t_submit;
 };
 
 struct ServeSlot {
   Launch L;
   std::vector<ServeQuery> running;
+  std::vector<char> reported;
+  unsigned long long *done_host = nullptr;  // host-mapped: per plan {raw count, 1 | flags << 8}
+  unsigned long long *done_dev = nullptr;   // the same memory as the device sees it
+  size_t off = 0, bytes = 0;                // stack region of the arena in use
+  double t_launch = 0;
   bool busy = false;
 };
 
@@ -2159,6 +2321,10 @@ struct tdfs_server {
   int max_batch;
   double budget;
   int32_t est_samples;
+  long long waves;     // per launch
+  int dstride;
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
   std::vector<ServeSlot> slots;
   std::deque<ServeQuery> waiting;
   uint64_t next_ticket = 1;
@@ -2168,23 +2334,63 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+static size_t serve_need_bytes(const tdfs_server *sv, int kmax) {
+  return (size_t)sv->waves * (size_t)wave_ints(kmax, sv->dstride) * sizeof(int);
+}
+
+static void serve_free(tdfs_server *sv) {
+  for (auto &sl : sv->slots) {
+    launch_free(sl.L);
+    if (sl.done_host) (void)hipHostFree(sl.done_host);
+    sl.done_host = nullptr;
+  }
+  if (sv->arena) (void)hipFree(sv->arena);
+  delete sv;
+}
+
 extern "C" int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slots, int32_t blocks_per_cu,
-                                 int32_t max_batch, double budget, tdfs_server **out) {
-  if (!ctx || !g || !out || slots < 1 || slots > 64 || max_batch < 1) return fail(TDFS_E_ARG, "bad server arguments");
+                                 int32_t max_batch, int64_t stack_mb, double budget, tdfs_server **out) {
+  if (!ctx || !g || !out || slots < 1 || slots > 64 || max_batch < 1 || max_batch > 4096)
+    return fail(TDFS_E_ARG, "bad server arguments");
+  if (g->device != ctx->device) return fail(TDFS_E_ARG, "graph and context on different devices");
   HIP_TRY(hipSetDevice(ctx->device));
   tdfs_server *sv = new tdfs_server();
   sv->ctx = ctx;
   sv->g = g;
-  sv->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 1;
+  sv->blocks_per_cu = blocks_per_cu > 0 ? std::min(blocks_per_cu, ctx->blocks_per_cu) : 1;
   sv->max_batch = max_batch;
   sv->budget = budget;
   sv->est_samples = 4000;
+  sv->waves = (long long)ctx->cus * sv->blocks_per_cu * kWavesPerBlock;
+  sv->dstride = graph_dstride(g);
+  // default arena: every slot can run a launch of 7-vertex plans at once
+  sv->arena_bytes = stack_mb > 0 ? (size_t)stack_mb << 20 : (size_t)slots * serve_need_bytes(sv, kMaxK);
+  if (sv->arena_bytes < serve_need_bytes(sv, 3)) {
+    const size_t need = serve_need_bytes(sv, 3);
+    delete sv;
+    return fail(TDFS_E_BUDGET, "stack memory of %lld MiB holds no launch (3-vertex plans need %zu bytes)",
+                (long long)stack_mb, need);
+  }
+  if (hipMalloc(&sv->arena, sv->arena_bytes) != hipSuccess) {
+    const size_t b = sv->arena_bytes;
+    delete sv;
+    return fail(TDFS_E_OOM, "serving stack arena of %zu bytes", b);
+  }
+  tdfs_tuning t = ctx->tune;
+  fill_defaults(&t);
   sv->slots.resize(slots);
   for (auto &sl : sv->slots) {
     int rc = launch_init(sl.L, ctx->device);
+    // the ring, control block and plan table are fixed here; stacks come from the arena
+    sl.L.wbuf_external = true;
+    if (!rc) rc = launch_reserve(sl.L, max_batch, 0, 0, 1ull << t.queue_log2);
+    if (!rc && hipHostMalloc(&sl.done_host, sizeof(unsigned long long) * 2 * max_batch,
+                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+      rc = fail(TDFS_E_OOM, "completion records");
+    if (!rc && hipHostGetDevicePointer((void **)&sl.done_dev, sl.done_host, 0) != hipSuccess)
+      rc = fail(TDFS_E_HIP, "completion records: no device address");
     if (rc) {
-      for (auto &x : sv->slots) launch_free(x.L);
-      delete sv;
+      serve_free(sv);
       return rc;
     }
   }
@@ -2192,11 +2398,16 @@ extern "C" int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slo
   return TDFS_OK;
 }
 
-extern "C" int tdfs_serve_submit(tdfs_server *sv, const tdfs_plan *plan, uint64_t *ticket) {
+extern "C" int tdfs_serve_submit_range(tdfs_server *sv, const tdfs_plan *plan, int64_t edge_begin, int64_t edge_end,
+                                       uint64_t *ticket) {
   if (!sv || !plan || !ticket) return fail(TDFS_E_ARG, "null argument");
   int rc = check_plan(*plan);
   if (rc) return rc;
   if (plan->labeled && !sv->g->labeled) return fail(TDFS_E_ARG, "labeled query on an unlabeled graph");
+  const size_t need = serve_need_bytes(sv, plan->k);
+  if (need > sv->arena_bytes)
+    return fail(TDFS_E_BUDGET, "query refused: its DFS stacks need %zu bytes, the server holds %zu", need,
+                sv->arena_bytes);
   if (sv->budget > 0) {
     double est[3];
     rc = tdfs_graph_estimate(sv->g, plan, sv->est_samples, 1, sv->budget, est);
@@ -2207,73 +2418,175 @@ extern "C" int tdfs_serve_submit(tdfs_server *sv, const tdfs_plan *plan, uint64_
   ServeQuery q;
   q.ticket = sv->next_ticket++;
   q.plan = *plan;
+  q.lo = edge_begin;
+  q.hi = edge_end;
+  q.view = (sv->g->ordered && prune_is_sound(*plan)) ? 1 : 0;
   q.t_submit = now_s();
   sv->waiting.push_back(q);
   *ticket = q.ticket;
   return TDFS_OK;
 }
 
-// Collects finished slots into (tickets, results, latency_s) up to cap and
-// starts idle slots on waiting queries.  *nout = entries written.
+extern "C" int tdfs_serve_submit(tdfs_server *sv, const tdfs_plan *plan, uint64_t *ticket) {
+  return tdfs_serve_submit_range(sv, plan, 0, -1, ticket);
+}
+
+// First free region of the arena of at least `bytes` (regions of busy slots
+// are taken), or SIZE_MAX.
+static size_t arena_fit(const tdfs_server *sv, size_t bytes) {
+  std::vector<std::pair<size_t, size_t>> used;
+  for (const auto &sl : sv->slots)
+    if (sl.busy) used.emplace_back(sl.off, sl.bytes);
+  std::sort(used.begin(), used.end());
+  size_t at = 0;
+  for (const auto &u : used) {
+    if (u.first >= at + bytes) return at;
+    at = std::max(at, u.first + u.second);
+  }
+  return sv->arena_bytes >= at + bytes ? at : SIZE_MAX;
+}
+
+// The result of running query i of a slot from its completion record
+// (or, done_rec false, from the launch's final counts).
+static void serve_result(const ServeSlot &sl, size_t i, bool done_rec, tdfs_result &r) {
+  std::memset(&r, 0, sizeof(r));
+  unsigned long long f = 0;
+  if (done_rec) {
+    r.raw = __atomic_load_n(&sl.done_host[2 * i + 0], __ATOMIC_RELAXED);
+    f = __atomic_load_n(&sl.done_host[2 * i + 1], __ATOMIC_RELAXED) >> 8;
+  } else {
+    r.raw = sl.L.h_counts[i];
+    f = sl.L.h_stats[4 * i + 3];
+  }
+  r.flags = (uint32_t)(f & 1ull);
+  if (f & 2ull) r.flags |= TDFS_FAILED;  // raw count overflowed 64 bits
+  r.kernel_ms = (now_s() - sl.t_launch) * 1e3;
+  const tdfs_plan &p = sl.running[i].plan;
+  if (p.labeled) {
+    r.count = r.raw;
+  } else {
+    const unsigned __int128 prod = (unsigned __int128)r.raw * p.multiplicity;
+    if (prod >> 64) r.flags |= TDFS_FAILED;
+    r.count = (uint64_t)prod;
+  }
+}
+
+// Collects finished queries into (tickets, results, latency_s), at most cap,
+// and starts idle slots on waiting queries.  *nout = entries written.  A
+// launch that fails on the device reports its unfinished queries with
+// TDFS_FAILED and frees its slot; a launch that cannot be started puts its
+// queries back at the head of the queue and returns the error.
 extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *results, double *latency_s,
                                int32_t cap, int32_t *nout) {
   if (!sv || !nout || (cap > 0 && (!tickets || !results))) return fail(TDFS_E_ARG, "null argument");
+  *nout = 0;
   HIP_TRY(hipSetDevice(sv->ctx->device));
   int n = 0;
+  auto report = [&](ServeSlot &sl, size_t i, const tdfs_result &r, double t) {
+    tickets[n] = sl.running[i].ticket;
+    results[n] = r;
+    if (latency_s) latency_s[n] = t - sl.running[i].t_submit;
+    sl.reported[i] = 1;
+    n++;
+  };
   for (auto &sl : sv->slots) {
     if (!sl.busy) continue;
-    if ((int)sl.running.size() > cap - n) continue;  // no room to report this slot yet
+    // queries whose own work has ended
+    for (size_t i = 0; i < sl.running.size() && n < cap; i++) {
+      if (sl.reported[i]) continue;
+      if (!(__atomic_load_n(&sl.done_host[2 * i + 1], __ATOMIC_ACQUIRE) & 1ull)) continue;
+      tdfs_result r;
+      serve_result(sl, i, true, r);
+      report(sl, i, r, now_s());
+    }
     const hipError_t e = hipEventQuery(sl.L.ev1);
     if (e == hipErrorNotReady) continue;
-    if (e != hipSuccess) return fail(TDFS_E_HIP, "event query: %s", hipGetErrorString(e));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, sl.L.ev0, sl.L.ev1));
-    std::vector<tdfs_plan> plans;
-    for (auto &q : sl.running) plans.push_back(q.plan);
-    std::vector<tdfs_result> res(plans.size());
+    const bool failed = e != hipSuccess || sl.L.h_ctrl->error != 0;
+    size_t left = 0;
+    for (size_t i = 0; i < sl.running.size(); i++) left += !sl.reported[i];
+    if ((int)left > cap - n) continue;  // no room to report the rest of this slot yet
     const double t = now_s();
-    int rc = launch_collect(sl.L, plans.data(), (int)plans.size(), ms, res.data(), nullptr);
-    if (rc) return rc;
-    for (size_t i = 0; i < plans.size(); i++) {
-      tickets[n] = sl.running[i].ticket;
-      results[n] = res[i];
-      if (latency_s) latency_s[n] = t - sl.running[i].t_submit;
-      n++;
+    for (size_t i = 0; i < sl.running.size(); i++) {
+      if (sl.reported[i]) continue;
+      tdfs_result r;
+      serve_result(sl, i, false, r);
+      if (failed) r.flags |= TDFS_FAILED;
+      report(sl, i, r, t);
     }
+    if (failed)
+      fail(TDFS_E_HIP, "serving launch failed: %s",
+           e != hipSuccess ? hipGetErrorString(e) : "device scheduler error");
     sl.running.clear();
+    sl.reported.clear();
     sl.busy = false;
   }
   for (auto &sl : sv->slots) {
     if (sl.busy || sv->waiting.empty()) continue;
+    // the longest FIFO prefix (same numbering, at most max_batch) whose stacks fit a free region
+    int kmax = 3;
+    size_t take = 0, off = SIZE_MAX, bytes = 0;
+    const int view = sv->waiting.front().view;
+    while (take < sv->waiting.size() && (int)take < sv->max_batch && sv->waiting[take].view == view) {
+      const int k2 = std::max(kmax, (int)sv->waiting[take].plan.k);
+      const size_t b2 = serve_need_bytes(sv, k2);
+      const size_t o2 = arena_fit(sv, b2);
+      if (o2 == SIZE_MAX) break;
+      kmax = k2;
+      off = o2;
+      bytes = b2;
+      take++;
+    }
+    if (take == 0) break;  // no stack memory free: wait for a running launch to end
     std::vector<tdfs_plan> plans;
-    while (!sv->waiting.empty() && (int)sl.running.size() < sv->max_batch) {
+    std::vector<int64_t> ranges;
+    for (size_t i = 0; i < take; i++) {
       sl.running.push_back(sv->waiting.front());
       plans.push_back(sv->waiting.front().plan);
+      ranges.push_back(sv->waiting.front().lo);
+      ranges.push_back(sv->waiting.front().hi);
       sv->waiting.pop_front();
     }
+    sl.reported.assign(take, 0);
+    sl.L.wbuf = reinterpret_cast<int *>(sv->arena + off);
+    sl.L.wbuf_ints = bytes / sizeof(int);
+    std::memset(sl.done_host, 0, sizeof(unsigned long long) * 2 * take);
     KParams P;
-    int rc = launch_prepare(sv->ctx, sl.L, sv->g, plans.data(), (int)plans.size(), -1, -1, false, sv->blocks_per_cu, P);
-    P.interleave = 0;  // a light query finishes its own sweep without waiting for heavy batch-mates
+    int rc = launch_prepare(sv->ctx, sl.L, sv->g, plans.data(), (int)plans.size(), -1, -1, false, sv->blocks_per_cu,
+                            P, ranges.data(), true);
     if (!rc) {
-      HIP_TRY(hipEventRecord(sl.L.ev0, sl.L.stream));
-      rc = launch_enqueue(sl.L, P, false);
-      if (!rc) HIP_TRY(hipEventRecord(sl.L.ev1, sl.L.stream));
+      P.pend = sl.L.cursors + 8;
+      P.done_host = sl.done_dev;
+      hipError_t e = hipEventRecord(sl.L.ev0, sl.L.stream);
+      if (e != hipSuccess) rc = fail(TDFS_E_HIP, "event record: %s", hipGetErrorString(e));
+      if (!rc) rc = launch_enqueue(sl.L, P, false);
+      if (!rc && (e = hipEventRecord(sl.L.ev1, sl.L.stream)) != hipSuccess)
+        rc = fail(TDFS_E_HIP, "event record: %s", hipGetErrorString(e));
     }
-    if (rc) return rc;
+    if (rc) {
+      // nothing was started: the queries go back to the head of the queue in order
+      for (size_t i = sl.running.size(); i-- > 0;) sv->waiting.push_front(sl.running[i]);
+      sl.running.clear();
+      sl.reported.clear();
+      *nout = n;
+      return rc;
+    }
+    sl.off = off;
+    sl.bytes = bytes;
+    sl.t_launch = now_s();
     sl.busy = true;
   }
   *nout = n;
   return TDFS_OK;
 }
 
-// out4 = {queries waiting, slots busy, slots, queries running}
+// out4 = {queries waiting, slots busy, slots, queries running (not yet collected)}
 extern "C" int tdfs_serve_status(const tdfs_server *sv, int64_t *out4) {
   if (!sv || !out4) return fail(TDFS_E_ARG, "null argument");
   int busy = 0;
   int64_t running = 0;
   for (const auto &sl : sv->slots) {
     busy += sl.busy;
-    running += (int64_t)sl.running.size();
+    for (char r : sl.reported) running += !r;
   }
   out4[0] = (int64_t)sv->waiting.size();
   out4[1] = busy;
@@ -2285,8 +2598,9 @@ extern "C" int tdfs_serve_status(const tdfs_server *sv, int64_t *out4) {
 extern "C" void tdfs_serve_destroy(tdfs_server *sv) {
   if (!sv) return;
   (void)hipSetDevice(sv->ctx->device);
-  for (auto &sl : sv->slots) launch_free(sl.L);
-  delete sv;
+  for (auto &sl : sv->slots)
+    if (sl.L.stream) (void)hipStreamSynchronize(sl.L.stream);
+  serve_free(sv);
 }
 
 extern "C" int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8) {

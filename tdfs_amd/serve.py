@@ -2,8 +2,11 @@
 
 `Server` wraps tdfs_serve_* (include/tdfs.h): queries are submitted as plans,
 wait in a FIFO, and start in groups of up to `max_batch` on whichever of the
-`slots` concurrent launches (own stream, own workspace, grid capped to
-`blocks_per_cu` workgroups per CU) is idle; results are collected by polling.
+`slots` concurrent launches (own stream and task ring, grid capped to
+`blocks_per_cu` workgroups per CU) is idle, once a region of the server's
+DFS-stack arena (`stack_mb`) large enough for the group's largest pattern is
+free.  Each query is collected as soon as its own work has ended (per-plan
+completion records in host-mapped memory), not when its launch ends.
 
 `run_stream` replays a seeded Poisson arrival process against a server and
 reports p50/p99 latency (submit -> result), throughput and matches/s.
@@ -27,15 +30,18 @@ from ._lib import check, i64p, lib
 
 class Server:
     def __init__(self, dev: Device, dg: DeviceGraph, slots: int = 4, blocks_per_cu: int = 1, max_batch: int = 32,
-                 budget: float = 0.0):
+                 budget: float = 0.0, stack_mb: int = 0):
         self.handle = C.c_void_p()
-        check(lib().tdfs_serve_create(dev.handle, dg.handle, slots, blocks_per_cu, max_batch, budget,
+        check(lib().tdfs_serve_create(dev.handle, dg.handle, slots, blocks_per_cu, max_batch, stack_mb, budget,
                                       C.byref(self.handle)), "serve create")
         self.cap = slots * max_batch
 
-    def submit(self, plan: Plan) -> int:
+    def submit(self, plan: Plan, edge_begin: int = 0, edge_end: int = -1) -> int:
+        """Queues a query (optionally over level-0 edges [edge_begin, edge_end)
+        of the numbering the device counts the plan on); returns its ticket."""
         t = C.c_uint64()
-        check(lib().tdfs_serve_submit(self.handle, C.byref(plan), C.byref(t)), "serve submit")
+        check(lib().tdfs_serve_submit_range(self.handle, C.byref(plan), edge_begin, edge_end, C.byref(t)),
+              "serve submit")
         return int(t.value)
 
     def poll(self) -> list[tuple[int, Result, float]]:
@@ -100,7 +106,9 @@ def query_stream(dg, count: int, seed: int, labeled_frac: float, nlabels: int, b
 
 def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progress: int = 0) -> dict:
     """Submits plans at seeded Poisson arrival times (rate queries/s), polls
-    until all are answered; latency = submit -> collected."""
+    until all are answered; latency = submit -> collected.  Queries flagged
+    incomplete (time limit) or failed are listed and left out of matches and
+    matches/s (their counts are not exact)."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(plans)))
     results = {}
@@ -126,8 +134,11 @@ def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progre
     lats = np.array([lat[i] for i in range(len(plans))])
     counts = [int(results[i].count) for i in range(len(plans))]
     incomplete = [i for i in range(len(plans)) if results[i].flags & 1]
+    failed = [i for i in range(len(plans)) if results[i].flags & 2]
+    exact = [c for i, c in enumerate(counts) if not results[i].flags & 3]
     return {
         "incomplete": incomplete,
+        "failed": failed,
         "rate_qps": rate,
         "queries": len(plans),
         "wall_s": wall,
@@ -135,8 +146,8 @@ def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progre
         "p50_ms": float(np.percentile(lats, 50) * 1e3),
         "p99_ms": float(np.percentile(lats, 99) * 1e3),
         "mean_ms": float(lats.mean() * 1e3),
-        "matches": int(sum(counts)),
-        "matches_per_s": sum(counts) / wall,
+        "matches": int(sum(exact)),
+        "matches_per_s": sum(exact) / wall,
         "counts": counts,
     }
 

@@ -28,7 +28,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import CSRGraph, estimate_work, gen_pattern, plan_pattern
+from . import CSRGraph, gen_pattern
 
 TRIANGLE = "v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n"
 K4 = "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n"
@@ -74,57 +74,119 @@ def graph(name: str) -> CSRGraph:
     raise ValueError(name)
 
 
-def patterns(name: str) -> dict[str, tuple[str, bool]]:
-    """Fixed pattern sets: name -> (.g text, labeled)."""
+def patterns(name: str, full: bool = False) -> dict[str, tuple[str, bool]]:
+    """Fixed pattern sets: name -> (.g text, labeled).  full: every pattern the
+    configuration names; otherwise only those timed by the benchmark (the
+    others are in OUT_OF_REACH)."""
     if name == "c1":
         return {"triangle": (TRIANGLE, False)}
     if name == "c2":
-        return {"4-clique": (K4, False), "diamond": (DIAMOND, False)}
-    if name == "c3":
+        out = {"4-clique": (K4, False), "diamond": (DIAMOND, False), "5-cycle": (C5, False)}
+    elif name == "c3":
         out = {}
-        for i, (nm, t) in enumerate((("house", HOUSE), ("5-clique", K5))):
+        for i, (nm, t) in enumerate((("house", HOUSE), ("5-clique", K5), ("6-path", P6), ("6-cycle", C6))):
             k = sum(1 for l in t.splitlines() if l.startswith("v"))
             labs = np.random.default_rng([3, i]).integers(0, 4, size=k)
             out[nm] = (with_pattern_labels(t, labs), True)
-        return out
-    raise ValueError(name)
-
-
-# patterns of the configurations that no single GPU finishes (estimates
-# capped at 1e12 list elements); recorded, not timed
-OUT_OF_REACH = {"c2": {"5-cycle": (C5, False)}, "c3": {"6-path": (P6, True), "6-cycle": (C6, True)}}
-
-
-def random_patterns(g_device: CSRGraph, count: int, seed: int, labeled_every: int, nlabels: int,
-                    budget: float, nsample: int = 4000, max_attempts: int = 50) -> list[dict]:
-    """Seeded connected patterns kept by the feasibility rule: pattern i has
-    k = 3 + i mod 5 vertices; its edge count (uniform in [k-1, k(k-1)/2]),
-    edges and labels are redrawn (attempt counter in the seed) until the
-    estimate fits the budget; after every max_attempts rejected draws the
-    vertex count drops by one (never below 3).  g_device must be in the
-    numbering the device counts on (degree_order)."""
-    out = []
-    for i in range(count):
-        labeled = (i % labeled_every) == labeled_every - 1 if labeled_every > 1 else True
-        k_req = 3 + i % 5
-        k = k_req
-        attempt = 0
-        while True:
-            if attempt > 0 and attempt % max_attempts == 0 and k > 3:
-                k -= 1  # no draw of this size fits the budget: one vertex fewer
-            rng = np.random.default_rng([seed, i, attempt])
-            m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
-            text = gen_pattern(k, m, nlabels if labeled else 0, int(rng.integers(1 << 62)))
-            plan = plan_pattern(text, labeled)
-            est = estimate_work(g_device, plan, nsample, 1, budget)
-            if not est["capped"] and est["cost"] <= budget:
-                out.append({"index": i, "attempt": attempt, "k_requested": k_req, "k": k, "m": m,
-                            "labeled": labeled, "text": text, "est_cost": est["cost"], "est_raw": est["raw"]})
-                break
-            attempt += 1
+    else:
+        raise ValueError(name)
+    if not full:
+        out = {k: v for k, v in out.items() if k not in OUT_OF_REACH.get(name, ())}
     return out
 
 
-def c4_patterns(g_device: CSRGraph, budget: float = 1e10) -> list[dict]:
-    """64 patterns, seed 4, every other one labeled (4 labels)."""
-    return random_patterns(g_device, 64, 4, 2, 4, budget)
+# patterns of the configurations that no single GPU finishes in the
+# benchmark's time (their full counts are not timed; parity on windows only)
+OUT_OF_REACH = {"c2": ("5-cycle",), "c3": ("6-path", "6-cycle")}
+
+
+def canonical_key(text: str, labeled: bool) -> tuple:
+    """Isomorphism class of a pattern: the lexicographically smallest
+    (labels, upper-triangle adjacency) over all vertex permutations (labels
+    count only when labeled)."""
+    from itertools import permutations
+
+    labs, edges = [], []
+    for line in text.splitlines():
+        f = line.split()
+        if f and f[0] == "v":
+            labs.append(int(f[2]) if labeled else 1)
+        elif f and f[0] == "e":
+            edges.append((int(f[1]), int(f[2])))
+    k = len(labs)
+    A = np.zeros((k, k), dtype=np.int8)
+    for a, b in edges:
+        A[a, b] = A[b, a] = 1
+    iu = np.triu_indices(k, 1)
+    best = None
+    for p in permutations(range(k)):
+        p = list(p)
+        key = (tuple(labs[j] for j in p), tuple(A[np.ix_(p, p)][iu].tolist()))
+        if best is None or key < best:
+            best = key
+    return (k,) + best
+
+
+def slot_edges(seed: int, i: int, k: int, redraw: int) -> int:
+    """Edge count of slot i: uniform in [k-1, k(k-1)/2] (min-connected to
+    complete); redraw r > 0 only after 8 r rejected draws at the earlier count."""
+    return int(np.random.default_rng([seed, i, 10_000 + redraw]).integers(k - 1, k * (k - 1) // 2 + 1))
+
+
+def slot_draw(seed: int, i: int, attempt: int, k: int, m: int, nlabels: int) -> str:
+    """Attempt `attempt` of slot i: a connected pattern with k vertices and m
+    edges (labels uniform in [0, nlabels) when nlabels > 0), in matching order."""
+    return gen_pattern(k, m, nlabels, int(np.random.default_rng([seed, i, attempt]).integers(1 << 62)))
+
+
+REDRAW_EVERY = 8
+
+
+def select_patterns(slots, seed: int, nlabels: int, measure, max_attempts: int = 96, progress=None):
+    """The measured feasibility rule.  Slot i = (k, labeled) keeps the first
+    attempt whose pattern (a) is not isomorphic (with labels, when labeled) to
+    a pattern already kept and (b) finishes alone within the time gate:
+    measure(text, labeled) -> (finished, ms).  The edge count is drawn
+    uniformly per slot and kept for REDRAW_EVERY attempts before it is drawn
+    again; after max_attempts // 2 rejected draws the slot takes one vertex
+    fewer.  Returns (kept patterns, every attempt with its outcome)."""
+    kept, keys, trials = [], set(), []
+    for i, (k_req, labeled) in enumerate(slots):
+        for attempt in range(max_attempts):
+            # after max_attempts // 2 rejected draws the slot falls back to one vertex fewer
+            k = k_req if attempt < max_attempts // 2 or k_req == 3 else k_req - 1
+            m = slot_edges(seed, i, k, attempt // REDRAW_EVERY)
+            text = slot_draw(seed, i, attempt, k, m, nlabels if labeled else 0)
+            key = canonical_key(text, labeled)
+            rec = {"index": i, "attempt": attempt, "k_requested": k_req, "k": k, "m": m, "labeled": bool(labeled),
+                   "text": text}
+            if key in keys:
+                rec["outcome"] = "duplicate"
+            else:
+                ok, ms = measure(text, labeled)
+                rec["ms"] = round(float(ms), 3)
+                rec["outcome"] = "kept" if ok else "over gate"
+            trials.append(rec)
+            if progress:
+                progress(rec)
+            if rec["outcome"] == "kept":
+                keys.add(key)
+                kept.append({f: rec[f] for f in ("index", "attempt", "k_requested", "k", "m", "labeled", "text", "ms")})
+                break
+        else:
+            raise RuntimeError(f"slot {i} (k={k}, labeled={labeled}): no draw within {max_attempts} attempts")
+    return kept, trials
+
+
+def c4_slots() -> list[tuple[int, bool]]:
+    """64 slots: k = 3 + i mod 5 (13 each of 3..6, 12 of 7), odd i labeled."""
+    return [(3 + i % 5, i % 2 == 1) for i in range(64)]
+
+
+def c5_slots(count: int = 200) -> list[tuple[int, bool]]:
+    """Query i: k uniform in 3..7 and labeled with probability 0.3, seeded [5, i]."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng([5, i])
+        out.append((int(rng.integers(3, 8)), bool(rng.random() < 0.3)))
+    return out

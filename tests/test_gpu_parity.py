@@ -4,6 +4,8 @@ Every case prints its name before touching the GPU so that a failing run
 names it."""
 import json
 import math
+import os
+import time
 from pathlib import Path
 
 import numpy as np
@@ -17,8 +19,16 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
+_T0 = time.time()
+
+
 def say(*a):
-    print("[gpu-case]", *a, flush=True)
+    line = " ".join(str(x) for x in a)
+    print(f"[gpu-case +{time.time() - _T0:7.1f}s]", line, flush=True)
+    log = os.environ.get("TDFS_CASE_LOG")
+    if log:
+        with open(log, "a") as f:
+            f.write(f"+{time.time() - _T0:.1f}s {line}\n")
 
 
 @pytest.fixture(scope="module")
@@ -501,3 +511,81 @@ def test_labeled_plan_needs_labels(dev):
     dg = dev.upload(G)
     with pytest.raises(T.TdfsError):
         dev.count(dg, T.plan_pattern(fixture(8), 1))
+
+
+def test_arbitrary_label_values(dev):
+    """Graph labels are arbitrary int32 values (negative, above 255, sparse):
+    the upload maps them densely and pattern labels go through the same map,
+    so counts equal the oracle's on the raw values; a pattern label that no
+    vertex carries gives 0."""
+    base = T.CSRGraph.chung_lu(6000, 30000, 2.1, 500, 43)
+    values = np.array([-7, 5, 300, 1 << 20], dtype=np.int32)
+    rng = np.random.default_rng(43)
+    G = T.CSRGraph(base.rowptr, base.colidx, values[rng.integers(0, 4, size=base.n)])
+    dg = dev.upload(G)
+    for i in (1, 2, 5, 11):
+        text = fixture(i)
+        k = sum(1 for l in text.splitlines() if l.startswith("v"))
+        for trial in range(3):
+            labs = values[np.random.default_rng([43, i, trial]).integers(0, 4, size=k)]
+            say("label values", f"{i}.g", labs.tolist())
+            check(dev, dg, G, with_pattern_labels(text, labs), 1)
+        absent = with_pattern_labels(text, [77] + [int(values[0])] * (k - 1))
+        assert dev.count(dg, T.plan_pattern(absent, 1)).count == 0
+    dg.free()
+
+
+def test_serving_reports_each_query_when_its_work_ends(dev):
+    """A light query that shares a launch with a heavy one is collected while
+    the heavy one still runs (per-plan completion records), with its exact
+    count; the heavy one is collected later."""
+    from tdfs_amd import serve
+    G = T.CSRGraph.chung_lu(200000, 1500000, 2.1, 8000, 71)
+    dg = dev.upload(G)
+    heavy = T.plan_pattern("v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 4 1\n", 0)
+    light = T.plan_pattern("v 0 1\nv 1 1\nv 2 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\n", 0)
+    want = dev.count(dg, light).count
+    base = dev.tuning()
+    try:
+        dev.set_tuning(time_limit_ms=1500)
+        sv = serve.Server(dev, dg, slots=1, blocks_per_cu=1, max_batch=2)
+        th = sv.submit(heavy)
+        tl = sv.submit(light)
+        say("serving heavy + light in one launch")
+        seen = {}
+        t0 = time.time()
+        while len(seen) < 2:
+            for tk, r, lat in sv.poll():
+                seen[tk] = (r, lat, sv.status()["running"])
+            assert time.time() - t0 < 60
+        r_l, lat_l, running_after_light = seen[tl]
+        r_h, lat_h, _ = seen[th]
+        assert r_l.count == want and not r_l.flags
+        assert lat_l < lat_h
+        assert running_after_light >= 1  # the heavy query was still on the device
+        sv.close()
+    finally:
+        dev.set_tuning(time_limit_ms=base.time_limit_ms)
+
+
+def test_serving_admits_by_stack_memory(dev):
+    """A stack arena that holds one 4-vertex launch at a time still answers
+    every query exactly (launches wait for memory); a query whose stacks can
+    never fit is refused with TDFS_E_BUDGET."""
+    from tdfs_amd import serve
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.1, 2000, 63).with_labels(3, 4)
+    dg = dev.upload(G)
+    plans = [T.plan_pattern(fixture(i), i % 2) for i in (1, 2, 3, 5, 8, 11)]
+    alone = [dev.count(dg, p).count for p in plans]
+    waves = dev.info()["cus"] * 4
+    dstride = max(64, (G.dmax + 63) // 64 * 64)
+    need4 = waves * (2048 + 2 * dstride) * 4
+    sv = serve.Server(dev, dg, slots=3, blocks_per_cu=1, max_batch=2, stack_mb=-(-need4 // (1 << 20)))
+    with pytest.raises(T.TdfsError) as e:
+        sv.submit(T.plan_pattern(fixture(7), 0))  # 5 vertices: more stack than the arena holds
+    assert e.value.code == -7
+    four = [j for j, p in enumerate(plans) if p.k <= 4]
+    say("serving under a one-launch stack arena", len(four), "queries")
+    r = serve.run_stream(sv, [plans[j] for j in four], 500.0, 3)
+    assert r["counts"] == [alone[j] for j in four]
+    sv.close()

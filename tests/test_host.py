@@ -205,23 +205,62 @@ def test_cli_usage_without_gpu():
         assert r.returncode == 2 and "usage" in r.stderr
 
 
-def test_c4_pattern_set_is_reproducible():
-    """The committed 64-pattern set follows from its seeded rule (first
-    indices regenerated here; the graph is the c4 graph in device numbering)."""
+@pytest.mark.parametrize("fname,key,slots,seed,nlabels", [
+    ("c4_patterns.json", "patterns", "c4_slots", 4, 4),
+    ("c5_queries.json", "queries", "c5_slots", 5, 8),
+])
+def test_measured_pattern_sets_follow_their_rule(fname, key, slots, seed, nlabels):
+    """The committed c4 / c5 sets follow their seeded rule: replaying
+    workloads.select_patterns with the recorded measurements in place of the
+    GPU redraws every attempt's pattern, recomputes every duplicate, and keeps
+    exactly the committed patterns (sound matching order, no two isomorphic)."""
     import json
     from tdfs_amd import workloads as W
-    d = json.loads((GOLDEN / "c4_patterns.json").read_text())
-    pats = d["patterns"]
-    assert len(pats) == 64 and [p["index"] for p in pats] == list(range(64))
-    assert sum(p["labeled"] for p in pats) == 32
-    for p in pats:
+    d = json.loads((GOLDEN / fname).read_text())
+    measured = iter([t for t in d["trials"] if t["outcome"] != "duplicate"])
+
+    def measure(text, labeled):
+        t = next(measured)
+        assert t["text"] == text and t["labeled"] == labeled
+        return t["outcome"] == "kept", t["ms"]
+
+    kept, trials = W.select_patterns(getattr(W, slots)(), seed, nlabels, measure)
+    assert [t["outcome"] for t in trials] == [t["outcome"] for t in d["trials"]]
+    assert kept == d[key]
+    for p in kept:
         plan = T.plan_pattern(p["text"], p["labeled"])
         assert plan.k == p["k"] and plan.adj[2][1] == 1
-        assert p["est_cost"] <= 1e10
-    G = W.graph("c4")
-    D = G.relabeled(W.degree_order(G))
-    again = W.random_patterns(D, 6, 4, 2, 4, 1e10)  # same rule, first six indices
-    assert [q["text"] for q in again] == [p["text"] for p in pats[:6]]
+        assert p["ms"] <= d["gate_ms"]
+    keys = {W.canonical_key(p["text"], p["labeled"]) for p in kept}
+    assert len(keys) == len(kept)
+
+
+def test_canonical_key_is_the_isomorphism_class():
+    """Relabelled copies of a pattern share its key; patterns that differ by
+    one edge or one label do not."""
+    from tdfs_amd import workloads as W
+    rng = np.random.default_rng(3)
+    for j in range(20):
+        k = int(rng.integers(3, 8))
+        m = int(rng.integers(k - 1, k * (k - 1) // 2 + 1))
+        text = T.gen_pattern(k, m, 3, int(rng.integers(1 << 62)))
+        perm = rng.permutation(k)
+        lines = []
+        for line in text.splitlines():
+            f = line.split()
+            if f and f[0] == "v":
+                lines.append((perm[int(f[1])], f"v {{}} {f[2]}"))
+            elif f and f[0] == "e":
+                lines.append((k + j, f"e {perm[int(f[1])]} {perm[int(f[2])]} 1"))
+        vs = sorted((i, t) for i, t in lines if t.startswith("v"))
+        moved = "\n".join(t.format(i) for i, t in vs) + "\n" + "\n".join(t for i, t in lines if t.startswith("e"))
+        for labeled in (False, True):
+            assert W.canonical_key(moved, labeled) == W.canonical_key(text, labeled)
+    k4 = W.K4
+    assert W.canonical_key(k4, False) != W.canonical_key(W.DIAMOND, False)
+    relab = W.with_pattern_labels(k4, [0, 0, 0, 1])
+    assert W.canonical_key(relab, True) != W.canonical_key(W.with_pattern_labels(k4, [0, 0, 1, 1]), True)
+    assert W.canonical_key(relab, False) == W.canonical_key(k4, False)
 
 
 def test_query_stream_rule_terminates_and_is_seeded():
