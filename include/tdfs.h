@@ -100,6 +100,16 @@ typedef struct tdfs_tuning {
   int32_t idle_deadline_ms;  /* a wave that has waited this long for work while no busy wave reported
                                 progress ends the launch with a scheduler error (TDFS_E_HIP) instead of
                                 waiting on (default 60000; < 0: never) */
+  int32_t steal;             /* 1: an idle wave takes the back half of a sibling wave's shallowest open
+                                DFS level (same workgroup, LDS atomics; the reference's disabled
+                                trans_skt, src/gpu_match.cu:35-125); 2: off (default).  Either way idle
+                                waves also take donated tasks from the HBM ring */
+  int32_t merge_min;         /* > 0: an extension with two input lists that both hold at least this many
+                                candidates (neither more than 8x the other) intersects them by merge path
+                                (co-rank split of the merge into 8 steps per lane, tiles staged in LDS)
+                                instead of probing the shorter list (default 0: never).  A kernel variant
+                                of its own (4 or 5 waves per SIMD settings other than 5 run the default
+                                kernel's occupancy); it excludes stealing */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */
@@ -111,7 +121,7 @@ typedef struct tdfs_result {
   uint64_t donated;      /* tasks pushed to the task ring */
   double kernel_ms;      /* device time of the launch (hipEvents on the library stream) */
   uint32_t flags;        /* TDFS_INCOMPLETE: the time limit dropped part of this plan's work */
-  uint32_t reserved;
+  uint32_t stolen;       /* work taken by idle waves from sibling waves (tdfs_tuning.steal) */
 } tdfs_result;
 
 #define TDFS_INCOMPLETE 1u /* the time limit dropped part of this plan's work: count is a lower bound */

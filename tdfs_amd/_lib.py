@@ -54,6 +54,8 @@ class Tuning(C.Structure):
         ("waves_per_simd", C.c_int32),
         ("hoist_min", C.c_int32),
         ("idle_deadline_ms", C.c_int32),
+        ("steal", C.c_int32),
+        ("merge_min", C.c_int32),
     ]
 
 
@@ -66,7 +68,7 @@ class Result(C.Structure):
         ("donated", C.c_uint64),
         ("kernel_ms", C.c_double),
         ("flags", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("stolen", C.c_uint32),
     ]
 
 

@@ -54,6 +54,7 @@ constexpr int kWavesPerBlock = 4;    // 256-thread workgroups
 constexpr int kBlock = kWavesPerBlock * kWave;
 constexpr long long kSpinMax = 1ll << 24;  // bound on waits for a slot handshake
 constexpr int kMaxSegLabels = 64;          // label segments are built for at most this many labels
+constexpr int kTile = 512;                 // merge-path diagonals per round (8 per lane)
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
 struct DevPlan {
@@ -166,6 +167,9 @@ struct KParams {
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
+  int steal;                 // 1: idle waves take work from sibling waves of their workgroup (LDS atomics)
+  int merge_min;             // two-input extensions whose inputs both hold at least this many candidates
+                             // intersect by merge path (0: never)
   int hoist_min;             // build H(u) for a level of at least this many candidates (0: never)
   // per-plan completion (serving): pend[pid * 16] counts the plan's level-0
   // chunks and ring tasks in flight; when it drops to zero with the plan's
@@ -180,16 +184,22 @@ struct KParams {
 };
 
 struct WaveLds {
+  // per stack level: the untaken iterations [it, end) of the level's list
+  // and the list's generation, packed as (gen << 48) | (it << 24) | end; the
+  // owner claims iterations from the front and an idle sibling wave (same
+  // workgroup) may take the back half, both by compare-and-swap
+  unsigned long long ie[8];
   int M[8];
-  int it[8];
-  int end[8];
   int len[8];
+  int pid;      // plan of the work in progress
+  int readers;  // sibling waves copying this wave's lists (the owner does not overwrite them meanwhile)
   int hv[8];  // H(u) built for the current M(0..hfix(u))
   int hl[8];  // |H(u)|
   // wave-uniform input lists of the extension in progress
   const int *ulp[8];
   const unsigned *uhb[8];
   int uln[8];
+  int *tile;  // merge-path variant: this wave's 2 (kTile + 4) LDS ints (the next kTile + 1 of both lists)
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -201,6 +211,126 @@ __device__ __forceinline__ unsigned long long uni64(unsigned long long x) {
 }
 __device__ __forceinline__ unsigned mbcnt64(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// ---------------------------------------------------------------------------
+// Stack-level iteration ranges.  The owner wave and, with stealing on, idle
+// sibling waves of its workgroup update them with LDS atomics (workgroup
+// scope); without stealing, lane 0 of the owner reads and writes them
+// plainly.  Every helper is called by all lanes and returns wave-uniform
+// values.
+// ---------------------------------------------------------------------------
+constexpr int kIeMax = (1 << 24) - 1;  // list lengths and positions fit 24 bits
+__device__ __forceinline__ unsigned long long ie_make(unsigned long long gen, int it, int end) {
+  return (gen << 48) | ((unsigned long long)(unsigned)it << 24) | (unsigned long long)(unsigned)end;
+}
+__device__ __forceinline__ int ie_it(unsigned long long w) { return (int)((w >> 24) & kIeMax); }
+__device__ __forceinline__ int ie_end(unsigned long long w) { return (int)(w & kIeMax); }
+__device__ __forceinline__ unsigned long long ie_gen(unsigned long long w) { return w >> 48; }
+
+__device__ __forceinline__ unsigned long long ie_read(WaveLds &S, int l) {
+  unsigned long long w = 0;
+  if (lane_id() == 0) w = __hip_atomic_load(&S.ie[l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return uni64(w);
+}
+
+// Starts level l on a new list: iterations [it, end), next generation.
+// The list and the path it depends on are written before (release).
+__device__ __forceinline__ void level_set(WaveLds &S, int l, int it, int end, bool shared) {
+  if (shared) __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the lists are in L2 before a sibling may copy them
+  if (lane_id() == 0) {
+    const unsigned long long w = __hip_atomic_load(&S.ie[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(&S.ie[l], ie_make(ie_gen(w) + 1, it, end), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
+// Claims up to `want` iterations from the front of level l: returns the
+// first, n = how many (0: none left).
+__device__ __forceinline__ int level_claim(WaveLds &S, int l, int want, int &n, bool shared) {
+  int first = 0, got = 0;
+  if (lane_id() == 0) {
+    unsigned long long w = __hip_atomic_load(&S.ie[l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+      const int it = ie_it(w), end = ie_end(w);
+      if (it >= end) break;
+      const int c = min(want, end - it);
+      const unsigned long long nw = ie_make(ie_gen(w), it + c, end);
+      if (!shared) {
+        __hip_atomic_store(&S.ie[l], nw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (!__hip_atomic_compare_exchange_strong(&S.ie[l], &w, nw, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        continue;
+      }
+      first = it;
+      got = c;
+      break;
+    }
+  }
+  n = uni(got);
+  return uni(first);
+}
+
+// Returns the back part [from, to) of the owner's last claim of level l
+// (to is the level's front: only the owner moves it, and thieves take only
+// from the back of [to, end)).
+__device__ __forceinline__ void level_unclaim(WaveLds &S, int l, int from, int to, bool shared) {
+  if (lane_id() == 0) {
+    unsigned long long w = __hip_atomic_load(&S.ie[l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+      const unsigned long long nw = ie_make(ie_gen(w), from, max(ie_end(w), to));
+      if (!shared) {
+        __hip_atomic_store(&S.ie[l], nw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        break;
+      }
+      if (__hip_atomic_compare_exchange_strong(&S.ie[l], &w, nw, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+        break;
+    }
+  }
+}
+
+// Takes iterations from the back of level l for donation: all of [it, end)
+// (all = true) or at most `most` of them; returns the first taken, n = how many.
+__device__ __forceinline__ int level_take_back(WaveLds &S, int l, bool all, int most, int &n, bool shared) {
+  int first = 0, got = 0;
+  if (lane_id() == 0) {
+    unsigned long long w = __hip_atomic_load(&S.ie[l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+      const int it = ie_it(w), end = ie_end(w);
+      if (it >= end) break;
+      const int c = all ? end - it : min(most, end - it);
+      const unsigned long long nw = ie_make(ie_gen(w), it, end - c);
+      if (!shared) {
+        __hip_atomic_store(&S.ie[l], nw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (!__hip_atomic_compare_exchange_strong(&S.ie[l], &w, nw, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        continue;
+      }
+      first = end - c;
+      got = c;
+      break;
+    }
+  }
+  n = uni(got);
+  return uni(first);
+}
+
+// Before the owner overwrites a list: waits until no sibling is copying its
+// lists.  Returns false if the wait did not end (error flag set by caller).
+__device__ __forceinline__ bool wait_readers(WaveLds &S, bool shared) {
+  if (!shared) return true;
+  int ok = 1;
+  if (lane_id() == 0) {
+    long long spins = 0;
+    while (__hip_atomic_load(&S.readers, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) {
+      if (++spins > (1ll << 22)) {
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return uni(ok) != 0;
 }
 
 template <typename T>
@@ -222,6 +352,13 @@ __device__ __forceinline__ T fadd(T *p, T v) {
 template <typename T>
 __device__ __forceinline__ T ld_rlx(T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Adds, then waits until the add has been performed: a later atomic of this
+// wave (to another address) cannot overtake it.
+template <typename T>
+__device__ __forceinline__ void fadd_done(T *p, T v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 
@@ -296,6 +433,74 @@ __device__ __forceinline__ void emit_rows(const KParams &P, const WaveLds &S, in
 }
 
 // ---------------------------------------------------------------------------
+// Merge-path intersection of two sorted, duplicate-free lists A[0, na) and
+// B[0, nb): the merge of A and B (ties take A first) is cut into rounds of
+// kTile steps.  Each round stages the next kTile + 1 elements of both lists
+// in LDS with coalesced loads; lane l finds where diagonal 8l crosses the
+// staged lists (co-rank: binary search in LDS) and performs merge steps
+// 8l .. 8l+7 there.  Taking A[a] while B's head equals it is a match, so
+// every common element is found exactly once, by one lane.  Matches passing
+// keep(x) are counted and, when out is non-null, written in ascending order
+// (count, wave scan, then the same steps again writing).  Returns the count.
+// ---------------------------------------------------------------------------
+template <typename Keep>
+__device__ __forceinline__ unsigned merge_intersect(WaveLds &S, const int *A, int na, const int *B, int nb, int *out,
+                                                    Keep keep) {
+  const int lane = lane_id();
+  constexpr int per = kTile / kWave;
+  int i0 = 0, j0 = 0;
+  unsigned n = 0;
+  int *ta = S.tile, *tb = S.tile + kTile + 4;
+  auto corank = [&](int d) {  // A elements among the first d merge steps of the staged lists
+    int lo = max(0, d - kTile), hi = min(d, kTile);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ta[mid] <= tb[d - mid - 1]) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  while (i0 < na && j0 < nb) {
+    for (int t = lane; t <= kTile; t += kWave) {
+      ta[t] = i0 + t < na ? A[i0 + t] : INT_MAX;
+      tb[t] = j0 + t < nb ? B[j0 + t] : INT_MAX;
+    }
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the staged tile is written before any lane reads it
+    const int a0 = corank(lane * per);
+    auto steps = [&](int *dst) -> int {  // this lane's merge steps; writes matches to dst when non-null
+      int a = a0, b = lane * per - a0, c = 0;
+      for (int st = 0; st < per; st++) {
+        const int x = ta[a], y = tb[b];
+        if (x <= y) {
+          if (x == y && i0 + a < na && j0 + b < nb && keep(x)) {
+            if (dst) dst[c] = x;
+            c++;
+          }
+          a++;
+        } else {
+          b++;
+        }
+      }
+      return c;
+    };
+    const int c = steps(nullptr);
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    if (out && c > 0) steps(out + n + (incl - c));
+    n += (unsigned)__shfl(incl, kWave - 1);
+    const int aend = corank(kTile);
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // every lane is done with the tile before it is restaged
+    i0 += aend;
+    j0 += kTile - aend;
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
 // Candidate generation for pattern vertex u (2 <= u <= k-1) given M(0..u-1).
 // Returns |C(u)|; writes C(u) (ascending) to out unless out == nullptr.
 //
@@ -307,7 +512,7 @@ __device__ __forceinline__ void emit_rows(const KParams &P, const WaveLds &S, in
 // descriptors live in the wave's LDS slot (S.ulp / S.uhb / S.uln), which
 // keeps them out of the vector registers.
 // ---------------------------------------------------------------------------
-template <bool STATS>
+template <bool STATS, bool MERGE = false>
 __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u,
                            int *__restrict__ out, const int *lbase, const int *hbase, int mode, bool count_stats,
                            unsigned long long &balg, unsigned long long &next) {
@@ -414,7 +619,27 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   if (lo >= 0) start = lower_bound_i(pp, pn, lo + 1);
   if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
   unsigned n = 0;
-  if (start < stop) {
+  // merge path: two inputs, both long, neither more than 8x the other
+  int mo = -1, ms = 0, me = 0;
+  if (MERGE && P.merge_min > 0 && nl == 2 && stop - start >= P.merge_min && !P.emit_n) {
+    const int oi = 1 - pidx;
+    const int *ob = S.ulp[oi];
+    const int on = S.uln[oi];
+    ms = lo >= 0 ? lower_bound_i(ob, on, lo + 1) : 0;
+    me = hi != INT_MAX ? lower_bound_i(ob, on, hi) : on;
+    if (me - ms >= P.merge_min && me - ms <= 8 * (stop - start)) mo = oi;
+  }
+  if (start < stop && mo >= 0) {
+    const int lab = pl->label[u];
+    n = merge_intersect(S, pp + start, stop - start, S.ulp[mo] + ms, me - ms, out, [&](int x) {
+      bool ok = true;
+#pragma unroll
+      for (int j = 0; j < 6; j++)
+        if ((nemask >> j) & 1) ok = ok && (x != nev[j]);
+      if (label_check && ok) ok = (int)P.labels[x] == lab;
+      return ok;
+    });
+  } else if (start < stop) {
     if (leaf && nl == 1 && label_known && !P.emit_n) {
       // Count-only leaf over one list: |list ∩ (lo,hi)| minus the excluded
       // path vertices that lie in that window and in the list.
@@ -919,20 +1144,24 @@ __device__ __forceinline__ bool ring_take(const KParams &P, TaskRec &t) {
 
 // Donates the untaken iterations of the shallowest open level in
 // [floor, level], cut into about 2*idle pieces (at least split_chunk
-// iterations each, at most 256 pieces).  Returns the number of tasks pushed
-// (0 when the ring is full).  All lanes call it.
+// iterations each, at most 256 pieces).  Ring slots are reserved first, then
+// the iterations are taken from the level (a sibling may have stolen part of
+// them meanwhile: slots left over carry empty tasks, p = 0).  Returns the
+// number of tasks pushed (0 when the ring is full).  All lanes call it.
+template <bool STEAL>
 __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
                       const int *L0c, int idle) {
-  int ld = level;
+  constexpr bool shared = STEAL;
+  int ld = -1, rem = 0;
   for (int l = floor; l <= level; l++) {
-    if (S.it[l] < S.end[l]) {
+    const unsigned long long w = ie_read(S, l);
+    if (ie_it(w) < ie_end(w)) {
       ld = l;
+      rem = ie_end(w) - ie_it(w);
       break;
     }
   }
-  const int it = S.it[ld], end = S.end[ld];
-  const int rem = end - it;
-  if (rem <= 0) return 0;
+  if (ld < 0) return 0;
   int piece = 1, ntasks = rem;
   if (ld > 0) {
     const int want = max(1, min(256, 2 * idle));
@@ -944,7 +1173,11 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
   const unsigned long long pos0 = ring_reserve(P, ntasks);
   if (pos0 == ~0ull) return 0;
   // the tasks are in flight before any of them can be taken
-  if (P.pend && lane_id() == 0) fadd(&P.pend[(long long)pid * 16], (unsigned long long)ntasks);
+  if (P.pend && lane_id() == 0) fadd_done(&P.pend[(long long)pid * 16], (unsigned long long)ntasks);
+  // level > 0: every untaken iteration, in pieces; level 0: the last edge pairs, one task each
+  int got = 0;
+  const int a0 = level_take_back(S, ld, ld > 0, ntasks, got, shared);
+  const int real = ld > 0 ? (got + piece - 1) / piece : got;
   bool okall = true;
   for (int j = lane_id(); j < ntasks; j += kWave) {
     TaskRec t;
@@ -952,10 +1185,14 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
     t.pad0 = t.pad1 = t.pad2 = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++) t.M[q] = 0;
-    if (ld == 0) {
+    t.a = t.b = 0;
+    t.fresh = 0;
+    if (j >= real) {
+      t.p = 0;  // empty task
+    } else if (ld == 0) {
       t.p = 2;
-      t.M[0] = L0r[end - ntasks + j];
-      t.M[1] = L0c[end - ntasks + j];
+      t.M[0] = L0r[a0 + j];
+      t.M[1] = L0c[a0 + j];
       t.a = 0;
       t.b = INT_MAX;
       t.fresh = 1;
@@ -964,9 +1201,8 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
 #pragma unroll
       for (int q = 0; q < 8; q++)
         if (q <= ld) t.M[q] = S.M[q];
-      t.a = it + j * piece;
-      t.b = min(t.a + piece, end);
-      t.fresh = 0;
+      t.a = a0 + j * piece;
+      t.b = min(t.a + piece, a0 + got);
     }
     okall = ring_publish(P, pos0 + j, t) && okall;
   }
@@ -975,10 +1211,7 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
       __hip_atomic_store(&P.ctrl->error, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (lane_id() == 0) fadd(&P.ctrl->q_avail, (long long)ntasks);
-  // every lane stores the same value; level 0 hands over its last ntasks edges
-  if (ld > 0) S.end[ld] = it;
-  else S.end[0] = end - ntasks;
-  return ntasks;
+  return real;
 }
 
 // After M(0..fhi) are fixed and the n candidates of the next level are
@@ -986,7 +1219,7 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
 // builds H(u) when n reaches hoist_min (else marks it unbuilt).  Returns false
 // when a built H(u) is empty: then no C(u) below this prefix is non-empty
 // (the timed kernel skips the subtree; the statistics kernel walks it).
-template <bool STATS>
+template <bool STATS, bool MERGE = false>
 __device__ __forceinline__ bool prepare_hoists(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int flo,
                                                int fhi, int umin, int n, const int *lbase, int *hbase) {
   const int ri = P.reuse ? 0 : 1;
@@ -997,7 +1230,7 @@ __device__ __forceinline__ bool prepare_hoists(const KParams &P, const DevPlan *
     int hv = 0, hn = 0;
     if (P.hoist_min > 0 && n >= P.hoist_min) {
       unsigned long long db = 0, dn = 0;
-      hn = (int)extend<STATS>(P, pl, S, u, hbase + (long long)(u - 3) * P.dstride, lbase, hbase, kBuildH, false, db,
+      hn = (int)extend<STATS, MERGE>(P, pl, S, u, hbase + (long long)(u - 3) * P.dstride, lbase, hbase, kBuildH, false, db,
                               dn);
       hv = 1;
       if (hn == 0) live = false;
@@ -1017,20 +1250,20 @@ __device__ __forceinline__ void clear_hoists(WaveLds &S) {
 // Iterative DFS from stack level `floor` (its list, iterator and end already
 // set).  Stack level L iterates C(L+1) (L >= 1) or the edge pairs (L = 0);
 // taking element x at level L fixes M(L+1) = x and computes C(L+2).
-template <bool STATS>
+template <bool STATS, bool STEAL, bool MERGE>
 __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
                     const int *L0r, const int *L0c, int *lbase, int *hbase, unsigned long long &cnt,
                     unsigned long long &balg, unsigned long long &next, unsigned long long &don,
                     long long *prof, long long t_wave) {
   const int k = pl->k;
+  constexpr bool shared = STEAL;
   int level = floor;
   long long t0 = wall_clock64();
   long long tprog = t0;
   const bool may_donate = P.timeout_ticks >= 0;
   for (;;) {
-    const int it = uni(S.it[level]);
-    const int end = uni(S.end[level]);
-    if (it >= end) {
+    const unsigned long long w = ie_read(S, level);
+    if (ie_it(w) >= ie_end(w)) {
       if (level == floor) return;
       level--;
       continue;
@@ -1057,23 +1290,27 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
         idle = uni(idle);
         if (idle > 0) {
           const long long tp = STATS ? clock64() : 0;
-          const int n = donate(P, S, pid, floor, level, L0r, L0c, idle);
+          const int n = donate<STEAL>(P, S, pid, floor, level, L0r, L0c, idle);
           if (STATS) prof[5] += clock64() - tp;
           don += (unsigned)n;
-          if (n > 0) continue;  // re-read this level: its end may have moved
+          if (n > 0) continue;  // re-read this level: its end has moved
         }
       }
     }
     if (level + 2 == k - 1 && P.leaf_batch) {
+      int n = 0;
+      const int it = level_claim(S, level, kWave, n, shared);
+      if (n == 0) continue;  // a sibling took the rest
       int taken = 0;
       const long long tp = STATS ? clock64() : 0;
-      cnt += leaf_batch<STATS>(P, pl, S, level, it, min(end, it + kWave), L0r, L0c, lbase, hbase, balg, next,
-                               taken);
+      cnt += leaf_batch<STATS>(P, pl, S, level, it, it + n, L0r, L0c, lbase, hbase, balg, next, taken);
       if (STATS) prof[2] += clock64() - tp;
-      S.it[level] = it + taken;
+      if (taken < n) level_unclaim(S, level, it + taken, it + n, shared);
       continue;
     }
-    S.it[level] = it + 1;
+    int n1 = 0;
+    const int it = level_claim(S, level, 1, n1, shared);
+    if (n1 == 0) continue;
     if (level == 0) {
       S.M[0] = L0r[it];
       S.M[1] = L0c[it];
@@ -1084,17 +1321,22 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     const int mode = S.hv[u] ? kHoisted : kFull;
     const long long tp = STATS ? clock64() : 0;
     if (u == k - 1) {
-      cnt += extend<STATS>(P, pl, S, u, nullptr, lbase, hbase, mode, true, balg, next);
+      cnt += extend<STATS, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode, true, balg, next);
       if (STATS) prof[2] += clock64() - tp;
       continue;
     }
+    if (!wait_readers(S, shared)) {
+      if (lane_id() == 0) __hip_atomic_store(&P.ctrl->error, 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
     int *out = lbase + (long long)level * P.dstride;  // list of level+1
-    const int n = (int)extend<STATS>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
+    const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
     if (n > 0) {
       S.len[level + 1] = n;
-      S.end[level + 1] = n;
-      S.it[level + 1] = 0;
-      if (prepare_hoists<STATS>(P, pl, S, u - 1, u - 1, u + 1, n, lbase, hbase)) level++;
+      if (prepare_hoists<STATS, MERGE>(P, pl, S, u - 1, u - 1, u + 1, n, lbase, hbase)) {
+        level_set(S, level + 1, 0, n, shared);
+        level++;
+      }
     }
     if (STATS) prof[1] += clock64() - tp;
   }
@@ -1112,15 +1354,99 @@ __device__ __forceinline__ void unit_done(const KParams &P, int pid) {
   const DevPlan *pl = P.plans + pid;
   if (ld_rlx(&P.cursors[(long long)pid * 16]) < (unsigned long long)pl->e_span) return;
   const unsigned long long c = ld_rlx(&P.counts[pid]);
-  const unsigned long long f = ld_rlx(&P.stats[(long long)pid * 4 + 3]);
+  const unsigned long long f = ld_rlx(&P.stats[(long long)pid * 4 + 3]) & 0xffull;
   __hip_atomic_store(&P.done_host[pid * 2 + 0], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&P.done_host[pid * 2 + 1], 1ull | (f << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Intra-workgroup stealing.  An idle wave looks for the shallowest stack
+// level of a sibling wave of its workgroup with at least two untaken
+// iterations and takes their back half by one LDS compare-and-swap on the
+// sibling's packed range.  The path M(0..l) and the list lengths it needs
+// are read before the swap, which succeeds only if that range (and so the
+// path below it) did not change; the sibling's lists of stack levels 1..l
+// (or, at level 0, the edge pairs taken) are copied after it, while the
+// sibling's reader count keeps it from overwriting them.  The thief then
+// continues level l from its own copies; nothing is recomputed, so the
+// statistics equal those of the sibling finishing the work itself.
+// Returns the level taken (the DFS floor) or -1; pid gets the sibling's plan.
+__device__ __forceinline__ int steal_from_sibling(const KParams &P, WaveLds *lds, int wib, WaveLds &S, int *L0r,
+                                                  int *L0c, int *lbase, int &pid) {
+  const int lane = lane_id();
+  for (int d = 1; d < kWavesPerBlock; d++) {
+    const int vib = (wib + d) % kWavesPerBlock;
+    WaveLds &V = lds[vib];
+    for (int l = 0; l < kMaxK - 2; l++) {
+      const unsigned long long w = ie_read(V, l);
+      const int it = ie_it(w), end = ie_end(w);
+      if (end - it < 2) continue;
+      const int vpid = uni(__hip_atomic_load(&V.pid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (lane == 0) {
+        __hip_atomic_fetch_add(&V.readers, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (P.pend) fadd_done(&P.pend[(long long)vpid * 16], 1ull);  // in flight before the sibling can end its unit
+      }
+      int mv = 0, lv = 0;
+      if (lane < 8) {
+        mv = V.M[lane];
+        lv = V.len[lane];
+      }
+      const int mid = it + (end - it + 1) / 2;
+      int ok = 0;
+      if (lane == 0) {
+        unsigned long long exp = w;
+        ok = __hip_atomic_compare_exchange_strong(&V.ie[l], &exp, ie_make(ie_gen(w), it, mid), __ATOMIC_ACQ_REL,
+                                                  __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)
+                 ? 1
+                 : 0;
+      }
+      ok = uni(ok);
+      if (!ok) {
+        if (lane == 0) {
+          if (P.pend) fadd(&P.pend[(long long)vpid * 16], ~0ull);  // the sibling's own unit is still in flight
+          __hip_atomic_fetch_add(&V.readers, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        continue;
+      }
+      // copy the sibling's lists (agent-scope loads read L2, past this CU's L1)
+      const long long vgw = (long long)blockIdx.x * kWavesPerBlock + vib;
+      int *vwb = P.wbuf + vgw * P.wstride;
+      if (l == 0) {
+        for (int i = mid + lane; i < end; i += kWave) {
+          L0r[i] = __hip_atomic_load(vwb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          L0c[i] = __hip_atomic_load(vwb + kChunkMax + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      for (int lev = 1; lev <= l; lev++) {
+        const int n = __shfl(lv, lev);
+        int *src = vwb + 2 * kChunkMax + (long long)(lev - 1) * P.dstride;
+        int *dst = lbase + (long long)(lev - 1) * P.dstride;
+        for (int i = lane; i < n; i += kWave) dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      if (lane == 0) __hip_atomic_fetch_add(&V.readers, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // continue level l from the copies; the plan is known before the level
+      // opens (a third wave may steal from it right away)
+      if (lane < 8) {
+        S.M[lane] = lane <= l ? mv : 0;
+        S.len[lane] = lane <= l ? lv : 0;
+      }
+      S.pid = vpid;
+      for (int j = 0; j < l; j++) level_set(S, j, 0, 0, true);
+      level_set(S, l, mid, end, true);
+      pid = vpid;
+      return l;
+    }
+  }
+  return -1;
+}
+
 // The persistent match loop of one wave (body of the match kernels below).
-template <bool STATS>
+// STEAL: idle waves also take work from sibling waves (steal_from_sibling).
+// MERGE: long two-input extensions may intersect by merge path (LDS tiles).
+template <bool STATS, bool STEAL, bool MERGE = false>
 __device__ __forceinline__ void match_body(const KParams &P) {
   __shared__ WaveLds lds[kWavesPerBlock];
+  __shared__ int tiles[MERGE ? kWavesPerBlock * 2 * (kTile + 4) : 1];
   const int lane = lane_id();
   const int wib = uni(threadIdx.x / kWave);  // wave-uniform: the workspace pointers stay scalar
   const long long gw = (long long)blockIdx.x * kWavesPerBlock + wib;
@@ -1130,7 +1456,13 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   int *lbase = wb + 2 * kChunkMax;  // C(2..k-2): stack levels 1..k-3
   int *hbase = wb + P.hoff;         // H(3..k-1)
   WaveLds &S = lds[wib];
+  S.tile = MERGE ? tiles + wib * 2 * (kTile + 4) : nullptr;
   clear_hoists(S);
+  if (lane < 8) S.ie[lane] = 0;
+  S.readers = 0;
+  S.pid = 0;
+  constexpr bool shared = STEAL;
+  __syncthreads();  // every wave's ranges are valid before a sibling may read them
   Ctrl *C = P.ctrl;
   unsigned long long balg = 0, next = 0, don = 0;
   int stat_pid = -1;
@@ -1144,7 +1476,8 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   for (;;) {
     // ---- acquire work: ring first, then the edge cursors, then wait ----
     TaskRec t;
-    int kind = 0;
+    int kind = 0;  // 1 level-0 chunk, 2 ring task, 3 stolen from a sibling
+    int sfloor = -1;
     long long e0 = 0, e1 = 0;
     int pid = 0;
     if (lane == 0) fadd(&C->busy, 1);
@@ -1170,7 +1503,8 @@ __device__ __forceinline__ void match_body(const KParams &P) {
       // edges are handed out, then moves to the next plan with edges left
       while (!P.interleave && plans_left > 0) {
         if (lane == 0) {
-          if (P.pend) fadd(&P.pend[(long long)cur_plan * 16], 1ull);  // a chunk may be in flight
+          // a chunk may be in flight: counted before the grab can be seen
+          if (P.pend) fadd_done(&P.pend[(long long)cur_plan * 16], 1ull);
           pos = fadd(&P.cursors[(long long)cur_plan * 16], (unsigned long long)P.chunk);
         }
         pos = uni64(pos);
@@ -1227,13 +1561,34 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           if (backoff < 8) backoff <<= 1;
           int avail = 0;
           if (lane == 0) avail = ld_rlx(&C->q_avail) > 0;
-          if (!uni(avail)) continue;
-          if (lane == 0) fadd(&C->busy, 1);
-          if (ring_take(P, t)) {
-            kind = 2;
-            break;
+          if (uni(avail)) {
+            if (lane == 0) fadd(&C->busy, 1);
+            if (ring_take(P, t)) {
+              kind = 2;
+              break;
+            }
+            if (lane == 0) fadd(&C->busy, -1);
           }
-          if (lane == 0) fadd(&C->busy, -1);
+          if (shared) {
+            // a sibling may still be copying this wave's lists: wait before stealing into them
+            if (!wait_readers(S, true)) {
+              if (lane == 0) __hip_atomic_store(&C->error, 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              continue;
+            }
+            // Attempts do not count as busy (thousands of idle waves polling
+            // would keep the termination test from ever seeing zero); a
+            // sibling with work is busy itself, and the thief counts itself
+            // busy once it holds the stolen work.
+            sfloor = steal_from_sibling(P, lds, wib, S, L0r, L0c, lbase, pid);
+            if (sfloor >= 0) {
+              kind = 3;
+              if (lane == 0) {
+                fadd(&C->busy, 1);
+                atomicAdd(&P.stats[pid * 4 + 3], 1ull << 32);  // steals, counted per plan
+              }
+              break;
+            }
+          }
         }
         if (STATS) prof[3] += clock64() - tpi;
         if (kind == 0) break;
@@ -1244,6 +1599,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
       is_idle = false;
     }
     if (kind == 2) pid = t.pid;
+    S.pid = pid;  // before any level of this unit becomes visible to siblings
     const DevPlan *pl = P.plans + pid;
     if (STATS && stat_pid != pid) {
       if (stat_pid >= 0 && lane == 0) {
@@ -1259,7 +1615,10 @@ __device__ __forceinline__ void match_body(const KParams &P) {
       if (lane == 0) atomicOr(&P.stats[pid * 4 + 3], 1ull);  // dropped: count incomplete
     } else {
       int floor = -1;  // stack level the DFS starts from (one call site for both kinds of work)
-      if (kind == 1) {
+      const bool lists_free = kind == 3 || wait_readers(S, shared);  // before this unit overwrites them
+      if (!lists_free) {
+        if (lane == 0) __hip_atomic_store(&C->error, 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else if (kind == 1) {
         const long long tp = STATS ? clock64() : 0;
         const int n0 = filter_edges<STATS>(P, pl, e0, e1, L0r, L0c, balg);
         if (STATS) prof[0] += clock64() - tp;
@@ -1267,11 +1626,16 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           clear_hoists(S);
           if (STATS) next += (unsigned)n0;
           S.len[0] = n0;
-          S.end[0] = n0;
-          S.it[0] = 0;
+          level_set(S, 0, 0, n0, shared);
           floor = 0;
         }
-      } else {
+      } else if (kind == 3) {
+        // stolen: path and lists copied, level sfloor open (hoisted lists rebuilt here)
+        clear_hoists(S);
+        if (sfloor == 0 ||
+            prepare_hoists<STATS, MERGE>(P, pl, S, 1, sfloor, sfloor + 2, S.len[sfloor], lbase, hbase))
+          floor = sfloor;
+      } else if (t.p >= 2) {  // p = 0: an empty task
         const int p = t.p;
 #pragma unroll
         for (int j = 0; j < 8; j++)
@@ -1285,20 +1649,25 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           const bool last = u == p;
           const bool leaf = last && p == k - 1;
           int *o = leaf ? nullptr : lbase + (long long)(u - 2) * P.dstride;
-          const unsigned n = extend<STATS>(P, pl, S, u, o, lbase, hbase, kFull, last && fresh, balg, next);
+          const unsigned n = extend<STATS, MERGE>(P, pl, S, u, o, lbase, hbase, kFull, last && fresh, balg, next);
           if (leaf) cnt += n;
           else S.len[u - 1] = (int)n;
         }
         if (STATS) prof[4] += clock64() - tp;
         if (p < k - 1) {
           const int n = S.len[p - 1];
-          S.it[p - 1] = t.a;
-          S.end[p - 1] = min(t.b, n);
           clear_hoists(S);
-          if (n > 0 && prepare_hoists<STATS>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase)) floor = p - 1;
+          if (n > 0 && prepare_hoists<STATS, MERGE>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase)) {
+            level_set(S, p - 1, min(t.a, n), min(t.b, n), shared);
+            floor = p - 1;
+          }
         }
       }
-      if (floor >= 0) dfs<STATS>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
+      if (floor >= 0)
+        dfs<STATS, STEAL, MERGE>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
+      // a DFS cut short (time limit) leaves levels open: close them before the next unit's plan is published
+      if (STEAL)
+        for (int j = 0; j < kMaxK - 2; j++) level_set(S, j, 0, 0, true);
     }
     if (lane == 0) {
       if (cnt) {
@@ -1331,18 +1700,36 @@ __device__ __forceinline__ void match_body(const KParams &P) {
 }
 
 // Timed / statistics kernel: registers as the compiler allocates them
-// (4 waves per SIMD, no spills).
+// (3 waves per SIMD, no spills).
 template <bool STATS>
 __global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
-  match_body<STATS>(P);
+  match_body<STATS, false>(P);
 }
 
 // The same loop compiled for 4, 5, 6 or 8 waves per SIMD: more resident DFS
 // stacks to hide memory latency, at the cost of register spills.
-__global__ void __launch_bounds__(kBlock, 4) tdfs_match_kernel_w4(KParams P) { match_body<false>(P); }
-__global__ void __launch_bounds__(kBlock, 5) tdfs_match_kernel_w5(KParams P) { match_body<false>(P); }
-__global__ void __launch_bounds__(kBlock, 6) tdfs_match_kernel_w6(KParams P) { match_body<false>(P); }
-__global__ void __launch_bounds__(kBlock, 8) tdfs_match_kernel_w8(KParams P) { match_body<false>(P); }
+__global__ void __launch_bounds__(kBlock, 4) tdfs_match_kernel_w4(KParams P) { match_body<false, false>(P); }
+__global__ void __launch_bounds__(kBlock, 5) tdfs_match_kernel_w5(KParams P) { match_body<false, false>(P); }
+__global__ void __launch_bounds__(kBlock, 6) tdfs_match_kernel_w6(KParams P) { match_body<false, false>(P); }
+__global__ void __launch_bounds__(kBlock, 8) tdfs_match_kernel_w8(KParams P) { match_body<false, false>(P); }
+
+// The same kernels with intra-workgroup stealing compiled in (its code
+// raises register use, so it is a variant of its own).
+template <bool STATS>
+__global__ void __launch_bounds__(kBlock) tdfs_match_kernel_steal(KParams P) {
+  match_body<STATS, true>(P);
+}
+__global__ void __launch_bounds__(kBlock, 4) tdfs_match_kernel_w4_steal(KParams P) { match_body<false, true>(P); }
+__global__ void __launch_bounds__(kBlock, 5) tdfs_match_kernel_w5_steal(KParams P) { match_body<false, true>(P); }
+__global__ void __launch_bounds__(kBlock, 6) tdfs_match_kernel_w6_steal(KParams P) { match_body<false, true>(P); }
+__global__ void __launch_bounds__(kBlock, 8) tdfs_match_kernel_w8_steal(KParams P) { match_body<false, true>(P); }
+
+// And with the merge-path intersection compiled in (tdfs_tuning.merge_min).
+template <bool STATS>
+__global__ void __launch_bounds__(kBlock) tdfs_match_kernel_merge(KParams P) {
+  match_body<STATS, false, true>(P);
+}
+__global__ void __launch_bounds__(kBlock, 5) tdfs_match_kernel_w5_merge(KParams P) { match_body<false, false, true>(P); }
 
 // Sets bit y of row h for every neighbour y of hub vertex hubs[h].
 __global__ void tdfs_hub_build_kernel(const long long *rowptr, const int *colidx, const int *hubs, int nhubs,
@@ -1451,6 +1838,8 @@ struct tdfs_ctx {
   int blocks_per_cu;        // occupancy limit of the timed kernel
   int blocks_per_cu_stats;  // and of the statistics kernel
   int blocks_per_cu_w[9];   // and of the timed kernel compiled for w = 4, 5, 6, 8 waves per SIMD
+  int steal_blocks_per_cu, steal_blocks_per_cu_stats, steal_blocks_per_cu_w[9];  // the same with stealing
+  int merge_blocks_per_cu, merge_blocks_per_cu_stats, merge_blocks_per_cu_w5;     // and with merge path
   tdfs_tuning tune;
   int stats;
   Launch main;              // used by tdfs_count / tdfs_count_batch
@@ -1482,6 +1871,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->leaf_pair_cap = 4096;
   t->hoist_min = 2;
   t->idle_deadline_ms = 60000;
+  t->steal = 2;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1508,6 +1898,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->leaf_pair_cap < 64) t->leaf_pair_cap = 64;
   if (t->hoist_min == 0) t->hoist_min = d.hoist_min;
   if (t->idle_deadline_ms == 0) t->idle_deadline_ms = d.idle_deadline_ms;
+  if (t->steal != 1) t->steal = 2;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -1563,16 +1954,32 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
   c->wallclock_khz = khz > 0 ? khz : 100000;
-  int nb = 0, nb2 = 0, nbw[9] = {};
+  int nb = 0, nb2 = 0, nbw[9] = {}, sb = 0, sb2 = 0, sbw[9] = {};
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[4], tdfs_match_kernel_w4, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[5], tdfs_match_kernel_w5, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[6], tdfs_match_kernel_w6, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbw[8], tdfs_match_kernel_w8, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdfs_match_kernel<false>, kBlock, 0));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, tdfs_match_kernel<true>, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sbw[4], tdfs_match_kernel_w4_steal, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sbw[5], tdfs_match_kernel_w5_steal, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sbw[6], tdfs_match_kernel_w6_steal, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sbw[8], tdfs_match_kernel_w8_steal, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sb, tdfs_match_kernel_steal<false>, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sb2, tdfs_match_kernel_steal<true>, kBlock, 0));
   c->blocks_per_cu = std::max(1, nb);
   c->blocks_per_cu_stats = std::max(1, nb2);
   for (int w = 0; w < 9; w++) c->blocks_per_cu_w[w] = std::max(1, nbw[w]);
+  int mb = 0, mb2 = 0, mbw5 = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&mb, tdfs_match_kernel_merge<false>, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&mb2, tdfs_match_kernel_merge<true>, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&mbw5, tdfs_match_kernel_w5_merge, kBlock, 0));
+  c->merge_blocks_per_cu = std::max(1, mb);
+  c->merge_blocks_per_cu_stats = std::max(1, mb2);
+  c->merge_blocks_per_cu_w5 = std::max(1, mbw5);
+  c->steal_blocks_per_cu = std::max(1, sb);
+  c->steal_blocks_per_cu_stats = std::max(1, sb2);
+  for (int w = 0; w < 9; w++) c->steal_blocks_per_cu_w[w] = std::max(1, sbw[w]);
   int rc = launch_init(c->main, device);
   if (rc) {
     launch_free(c->main);
@@ -2027,12 +2434,19 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   const int chunk = t.chunk_edges;
   const long long span = edge_end - edge_begin;
   const int dstride = graph_dstride(g);
+  if (g->dmax > kIeMax) return fail(TDFS_E_ARG, "maximum degree %d beyond %d", g->dmax, kIeMax);
   int kmax = 3;
   for (int i = 0; i < nplans; i++) kmax = std::max(kmax, (int)plans[i].k);
   const long long wstride = wave_ints(kmax, dstride);
   const int wv = stats ? 0 : t.waves_per_simd;
-  const int occ = stats ? ctx->blocks_per_cu_stats
-                        : (wv > 0 ? ctx->blocks_per_cu_w[wv] : ctx->blocks_per_cu);
+  const bool st = t.steal == 1 && t.merge_min <= 0;  // merge path and stealing are separate variants
+  const bool mg = t.merge_min > 0;
+  const int occ = mg ? (stats ? ctx->merge_blocks_per_cu_stats
+                              : (wv == 5 ? ctx->merge_blocks_per_cu_w5 : ctx->merge_blocks_per_cu))
+                : st ? (stats ? ctx->steal_blocks_per_cu_stats
+                              : (wv > 0 ? ctx->steal_blocks_per_cu_w[wv] : ctx->steal_blocks_per_cu))
+                     : (stats ? ctx->blocks_per_cu_stats
+                              : (wv > 0 ? ctx->blocks_per_cu_w[wv] : ctx->blocks_per_cu));
   int bpc = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, occ) : occ;
   if (blocks_cap > 0) bpc = std::min(bpc, blocks_cap);
   int grid = ctx->cus * bpc;
@@ -2109,6 +2523,8 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
   P.leaf_pair_cap = t.leaf_pair_cap;
+  P.merge_min = t.merge_min > 0 ? t.merge_min : 0;
+  P.steal = t.steal == 1 && !P.merge_min ? 1 : 0;
   P.hoist_min = t.hoist_min > 0 ? t.hoist_min : 0;
   return TDFS_OK;
 }
@@ -2120,7 +2536,27 @@ static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
   hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, L.stream, L.ctrl, L.counts, L.stats,
                      P.nplans, L.q_seq, L.q_cap, L.prof, L.cursors);
   HIP_TRY(hipGetLastError());
-  if (stats)
+  if (P.merge_min) {
+    if (stats)
+      hipLaunchKernelGGL(tdfs_match_kernel_merge<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else if (P.variant == 5)
+      hipLaunchKernelGGL(tdfs_match_kernel_w5_merge, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else
+      hipLaunchKernelGGL(tdfs_match_kernel_merge<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+  } else if (P.steal) {
+    if (stats)
+      hipLaunchKernelGGL(tdfs_match_kernel_steal<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else if (P.variant == 8)
+      hipLaunchKernelGGL(tdfs_match_kernel_w8_steal, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else if (P.variant == 6)
+      hipLaunchKernelGGL(tdfs_match_kernel_w6_steal, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else if (P.variant == 5)
+      hipLaunchKernelGGL(tdfs_match_kernel_w5_steal, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else if (P.variant == 4)
+      hipLaunchKernelGGL(tdfs_match_kernel_w4_steal, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+    else
+      hipLaunchKernelGGL(tdfs_match_kernel_steal<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
+  } else if (stats)
     hipLaunchKernelGGL(tdfs_match_kernel<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
   else if (P.variant == 8)
     hipLaunchKernelGGL(tdfs_match_kernel_w8, dim3(L.grid), dim3(kBlock), 0, L.stream, P);
@@ -2159,6 +2595,7 @@ static int launch_collect(const Launch &L, const tdfs_plan *plans, int nplans, d
     r.extensions = L.h_stats[4 * i + 1];
     r.donated = L.h_stats[4 * i + 2];
     r.flags = (uint32_t)(L.h_stats[4 * i + 3] & 1ull);
+    r.stolen = (uint32_t)(L.h_stats[4 * i + 3] >> 32);
     r.kernel_ms = ms;
     if (L.h_stats[4 * i + 3] & 2ull) return fail(TDFS_E_OVERFLOW, "raw count of plan %d overflows 64 bits", i);
     if (plans[i].labeled) {
@@ -2456,7 +2893,7 @@ static void serve_result(const ServeSlot &sl, size_t i, bool done_rec, tdfs_resu
     f = __atomic_load_n(&sl.done_host[2 * i + 1], __ATOMIC_RELAXED) >> 8;
   } else {
     r.raw = sl.L.h_counts[i];
-    f = sl.L.h_stats[4 * i + 3];
+    f = sl.L.h_stats[4 * i + 3] & 0xffull;
   }
   r.flags = (uint32_t)(f & 1ull);
   if (f & 2ull) r.flags |= TDFS_FAILED;  // raw count overflowed 64 bits

@@ -100,6 +100,9 @@ def patterns(name: str, full: bool = False) -> dict[str, tuple[str, bool]]:
 OUT_OF_REACH = {"c2": ("5-cycle",), "c3": ("6-path", "6-cycle")}
 
 
+_PERMS: dict[int, np.ndarray] = {}
+
+
 def canonical_key(text: str, labeled: bool) -> tuple:
     """Isomorphism class of a pattern: the lexicographically smallest
     (labels, upper-triangle adjacency) over all vertex permutations (labels
@@ -114,22 +117,22 @@ def canonical_key(text: str, labeled: bool) -> tuple:
         elif f and f[0] == "e":
             edges.append((int(f[1]), int(f[2])))
     k = len(labs)
-    A = np.zeros((k, k), dtype=np.int8)
+    A = np.zeros((k, k), dtype=np.int64)
     for a, b in edges:
         A[a, b] = A[b, a] = 1
+    if k not in _PERMS:
+        _PERMS[k] = np.array(list(permutations(range(k))), dtype=np.int64)
+    P = _PERMS[k]
     iu = np.triu_indices(k, 1)
-    best = None
-    for p in permutations(range(k)):
-        p = list(p)
-        key = (tuple(labs[j] for j in p), tuple(A[np.ix_(p, p)][iu].tolist()))
-        if best is None or key < best:
-            best = key
-    return (k,) + best
+    rows = np.concatenate([np.asarray(labs, dtype=np.int64)[P], A[P[:, :, None], P[:, None, :]][:, iu[0], iu[1]]],
+                          axis=1)
+    best = rows[np.lexsort(rows.T[::-1])[0]]
+    return (k, tuple(best[:k].tolist()), tuple(best[k:].tolist()))
 
 
 def slot_edges(seed: int, i: int, k: int, redraw: int) -> int:
     """Edge count of slot i: uniform in [k-1, k(k-1)/2] (min-connected to
-    complete); redraw r > 0 only after 8 r rejected draws at the earlier count."""
+    complete); redraw r > 0 only after REDRAW_EVERY r rejected draws."""
     return int(np.random.default_rng([seed, i, 10_000 + redraw]).integers(k - 1, k * (k - 1) // 2 + 1))
 
 
@@ -139,42 +142,67 @@ def slot_draw(seed: int, i: int, attempt: int, k: int, m: int, nlabels: int) -> 
     return gen_pattern(k, m, nlabels, int(np.random.default_rng([seed, i, attempt]).integers(1 << 62)))
 
 
-REDRAW_EVERY = 8
+REDRAW_EVERY = 2
+PER_SIZE = 8
 
 
-def select_patterns(slots, seed: int, nlabels: int, measure, max_attempts: int = 96, progress=None):
+def size_order(k_req: int) -> list[int]:
+    """Sizes a slot tries in turn: the requested one, then smaller ones, then
+    larger ones (3..7)."""
+    return [k_req] + list(range(k_req - 1, 2, -1)) + list(range(k_req + 1, 8))
+
+
+def select_patterns(slots, seed: int, nlabels: int, measure, progress=None):
     """The measured feasibility rule.  Slot i = (k, labeled) keeps the first
     attempt whose pattern (a) is not isomorphic (with labels, when labeled) to
     a pattern already kept and (b) finishes alone within the time gate:
-    measure(text, labeled) -> (finished, ms).  The edge count is drawn
-    uniformly per slot and kept for REDRAW_EVERY attempts before it is drawn
-    again; after max_attempts // 2 rejected draws the slot takes one vertex
-    fewer.  Returns (kept patterns, every attempt with its outcome)."""
-    kept, keys, trials = [], set(), []
-    for i, (k_req, labeled) in enumerate(slots):
-        for attempt in range(max_attempts):
-            # after max_attempts // 2 rejected draws the slot falls back to one vertex fewer
-            k = k_req if attempt < max_attempts // 2 or k_req == 3 else k_req - 1
-            m = slot_edges(seed, i, k, attempt // REDRAW_EVERY)
-            text = slot_draw(seed, i, attempt, k, m, nlabels if labeled else 0)
-            key = canonical_key(text, labeled)
-            rec = {"index": i, "attempt": attempt, "k_requested": k_req, "k": k, "m": m, "labeled": bool(labeled),
-                   "text": text}
-            if key in keys:
-                rec["outcome"] = "duplicate"
-            else:
-                ok, ms = measure(text, labeled)
-                rec["ms"] = round(float(ms), 3)
-                rec["outcome"] = "kept" if ok else "over gate"
-            trials.append(rec)
-            if progress:
-                progress(rec)
-            if rec["outcome"] == "kept":
-                keys.add(key)
-                kept.append({f: rec[f] for f in ("index", "attempt", "k_requested", "k", "m", "labeled", "text", "ms")})
+    measure(text, labeled) -> (finished, ms).  The slot draws PER_SIZE
+    patterns of each size of size_order(k) in turn; the edge count is drawn
+    uniformly in [k-1, k(k-1)/2] and kept for REDRAW_EVERY draws; a pattern
+    isomorphic to one already measured over the gate is rejected unmeasured.  An
+    unlabeled slot that finds nothing at any size then draws labeled
+    patterns the same way: a hub-heavy graph has only a handful of unlabeled
+    connected shapes that finish within the gate (2 of 3 vertices, at most 6
+    of 4, a few dense ones of 5-6), too few for every unlabeled slot.
+    Returns (kept patterns, every attempt with its outcome)."""
+    kept, keys, over, trials = [], set(), set(), []
+    for i, (k_req, lab_req) in enumerate(slots):
+        order = size_order(k_req)
+        passes = [lab_req] if lab_req else [False, True]
+        done = False
+        attempt = 0
+        for labeled in passes:
+            for j in range(PER_SIZE * len(order)):
+                k = order[j // PER_SIZE]
+                m = slot_edges(seed, i, k, attempt // REDRAW_EVERY)
+                text = slot_draw(seed, i, attempt, k, m, nlabels if labeled else 0)
+                key = canonical_key(text, labeled)
+                rec = {"index": i, "attempt": attempt, "k_requested": k_req, "k": k, "m": m,
+                       "labeled_requested": bool(lab_req), "labeled": bool(labeled), "text": text}
+                attempt += 1
+                if key in keys:
+                    rec["outcome"] = "duplicate"
+                elif key in over:
+                    rec["outcome"] = "known over gate"  # the same pattern was measured before
+                else:
+                    ok, ms = measure(text, labeled)
+                    rec["ms"] = round(float(ms), 3)
+                    rec["outcome"] = "kept" if ok else "over gate"
+                    if not ok:
+                        over.add(key)
+                trials.append(rec)
+                if progress:
+                    progress(rec)
+                if rec["outcome"] == "kept":
+                    keys.add(key)
+                    kept.append({f: rec[f] for f in ("index", "attempt", "k_requested", "k", "m",
+                                                     "labeled_requested", "labeled", "text", "ms")})
+                    done = True
+                    break
+            if done:
                 break
-        else:
-            raise RuntimeError(f"slot {i} (k={k}, labeled={labeled}): no draw within {max_attempts} attempts")
+        if not done:
+            raise RuntimeError(f"slot {i} (k={k_req}, labeled={lab_req}): no pattern of any size within the gate")
     return kept, trials
 
 

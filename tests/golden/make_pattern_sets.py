@@ -4,7 +4,7 @@ tests/golden/c5_queries.json).
 
 c4: 64 slots, seed 4 (tdfs_amd.workloads.c4_slots: k = 3 + i mod 5, odd i
     labeled with 4 labels), on the Chung-Lu n=1.1M m=3.0M gamma=2.1 graph
-    (labels seed 3), gate 200 ms alone with the benchmark's c4 tuning.
+    (labels seed 3), gate 500 ms alone with the benchmark's c4 tuning.
 c5: 200 queries, seed 5 (workloads.c5_slots: k uniform in 3..7, 30% labeled
     with 8 labels), on the Chung-Lu n=4M m=34M gamma=2.3 graph, gate 100 ms
     alone with the default tuning.
@@ -29,7 +29,7 @@ import tdfs_amd as T  # noqa: E402
 from tdfs_amd import workloads as W  # noqa: E402
 
 SETS = {
-    "c4": dict(graph="c4", slots=W.c4_slots, seed=4, nlabels=4, gate_ms=200, tuned="c4"),
+    "c4": dict(graph="c4", slots=W.c4_slots, seed=4, nlabels=4, gate_ms=500, tuned="c4"),
     "c5": dict(graph="c5", slots=W.c5_slots, seed=5, nlabels=8, gate_ms=100, tuned=None),
 }
 
@@ -66,9 +66,11 @@ def main() -> int:
         "gate_ms": gate,
         "tuning": tuning or "defaults",
         ("patterns" if which == "c4" else "queries"): kept,
-        "trials": trials,
+        # the text of an attempt is kept where it was measured (a replay regenerates the rest)
+        "trials": [t if t["outcome"] in ("kept", "over gate") else {f: v for f, v in t.items() if f != "text"}
+                   for t in trials],
     }
-    out_path.write_text(json.dumps(out, indent=1))
+    out_path.write_text(json.dumps(out, separators=(",", ":")))
     print("wrote", out_path, len(kept), "patterns,", len(trials), "attempts", flush=True)
     return 0
 

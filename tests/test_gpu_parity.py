@@ -176,6 +176,17 @@ def test_scheduling_modes_are_exact(dev):
         dict(hoist_min=1, reuse=2),
         dict(hoist_min=1, leaf_batch=2),
         dict(hoist_min=64),
+        dict(steal=1),
+        dict(steal=1, timeout_us=1, donate_levels=5),
+        dict(steal=1, waves_per_simd=5),
+        dict(steal=1, leaf_batch=2),
+        dict(steal=1, hoist_min=1, timeout_us=1),
+        dict(steal=1, chunk_edges=1024, timeout_us=-1),
+        dict(steal=1, reuse=2, chunk_edges=512),
+        dict(merge_min=1),
+        dict(merge_min=8, waves_per_simd=5),
+        dict(merge_min=1, hoist_min=-1, leaf_batch=2),
+        dict(merge_min=4, timeout_us=1, donate_levels=5),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
@@ -183,6 +194,28 @@ def test_scheduling_modes_are_exact(dev):
                 say("sched", f"{i}.g", labeled, mode)
                 r = check(dev, dg, G, fixture(i), labeled, stats=mode.get("reuse") != 2, **mode)
                 dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_ if f != "reserved"})
+
+
+def test_stealing_takes_work_and_is_exact(dev):
+    """With ring donation off and 1,024-edge chunks, idle waves can only get
+    work from sibling waves of their workgroup: some work is stolen, and
+    counts, algorithmic bytes and extensions still equal the oracle's (the
+    thief copies the sibling's lists, it recomputes nothing)."""
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.1, 2000, 77).with_labels(3, 7)
+    dg = dev.upload(G)
+    base_t = dev.tuning()
+    stolen = 0
+    for i in (1, 2, 3, 7, 11, 13):
+        for labeled in (0, 1):
+            for mode in (dict(steal=1, timeout_us=-1, chunk_edges=1024),
+                         dict(steal=1, timeout_us=-1, chunk_edges=1024, waves_per_simd=5),
+                         dict(steal=1, chunk_edges=1024, leaf_batch=2, hoist_min=1)):
+                say("steal", f"{i}.g", labeled, mode)
+                r = check(dev, dg, G, fixture(i), labeled, stats=mode.get("waves_per_simd") is None, **mode)
+                stolen += r.stolen
+                dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_})
+    assert stolen > 0
+    dg.free()
 
 
 def test_many_labels_without_segments(dev):
@@ -285,7 +318,8 @@ def test_dense_core_long_shared_lists(dev):
     assert r.count >= core * (core - 1) * (core - 2) * (core - 3)
     for i in (1, 2, 7):
         for labeled in (0, 1):
-            for mode in (dict(), dict(leaf_pair_cap=64, timeout_us=1), dict(hoist_min=1)):
+            for mode in (dict(), dict(leaf_pair_cap=64, timeout_us=1), dict(hoist_min=1), dict(merge_min=16),
+                         dict(merge_min=1, leaf_batch=2)):
                 say("dense core", f"{i}.g", labeled, mode)
                 check(dev, dg, G, fixture(i), labeled, **mode)
                 dev.set_tuning(**{f: getattr(base_t, f) for f, _ in base_t._fields_ if f != "reserved"})
@@ -370,12 +404,24 @@ def test_batch_equals_alone(dev):
     batch = dev.count_batch(dg, plans)
     dev.set_tuning(cursor_order=2)
     batch2 = dev.count_batch(dg, plans)
+    # stealing between waves of different plans, both cursor orders
+    base = dev.tuning()
+    say("batch with stealing")
+    dev.set_tuning(steal=1, timeout_us=20, chunk_edges=256)
+    batch3 = dev.count_batch(dg, plans)
     dev.set_tuning(cursor_order=1)
+    batch4 = dev.count_batch(dg, plans)
+    say("batch with merge path")
+    dev.set_tuning(steal=2, merge_min=8)
+    batch5 = dev.count_batch(dg, plans)
+    dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+    dev.set_tuning(cursor_order=1, merge_min=0)
     for j, p in enumerate(plans):
         say("alone", j)
         alone = dev.count(dg, p)
-        assert alone.raw == batch[j].raw == batch2[j].raw, j
-        assert alone.alg_bytes == batch[j].alg_bytes == batch2[j].alg_bytes, j
+        for b in (batch, batch2, batch3, batch4, batch5):
+            assert alone.raw == b[j].raw, j
+            assert alone.alg_bytes == b[j].alg_bytes, j
 
 
 def test_graph_replays_match_eager(dev):
@@ -434,6 +480,14 @@ def test_serving_mix_equals_alone(dev):
         assert r["counts"] == alone, rate
         assert r["p99_ms"] >= r["p50_ms"] > 0
     sv.close()
+    base = dev.tuning()
+    dev.set_tuning(steal=1, timeout_us=20)  # completion records stay exact when work moves between waves
+    sv = serve.Server(dev, dg, slots=2, blocks_per_cu=1, max_batch=8)
+    say("serving stream with stealing")
+    r = serve.run_stream(sv, plans, 5000.0, 8)
+    assert r["counts"] == alone
+    sv.close()
+    dev.set_tuning(steal=base.steal, timeout_us=base.timeout_us)
 
 
 def test_serving_refuses_over_budget(dev):
