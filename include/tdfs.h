@@ -226,6 +226,22 @@ int tdfs_graph_estimate(const tdfs_graph *g, const tdfs_plan *plan, int32_t nsam
  * (edge_end < 0: all).  Blocks until done. */
 int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
                int64_t edge_end, tdfs_result *out);
+/* tdfs_count without any allocation at call time.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every buffer of the launch (control block, result word, plan,
+ * a task ring of 16,384 entries and the per-wave DFS stacks) lives in a
+ * static device array of 256 MiB that each GPU holds from library load until
+ * the process ends; its address is looked up once per device by
+ * tdfs_ctx_create.  The persistent grid shrinks to the workgroups whose DFS
+ * stacks fit (one wave needs 4 x (2,048 + 2 (k-3) x max degree) bytes); a plan
+ * of which not even one workgroup fits is refused with TDFS_E_BUDGET.  Same
+ * results as tdfs_count (tuning, statistics and edge range included).  All
+ * calls on one GPU share the array: calls that overlap on one GPU (two
+ * contexts or threads) race; the array keeps the last call's contents, and
+ * each launch clears what must start at zero. */
+int tdfs_count_noalloc(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
+                       int64_t edge_end, tdfs_result *out);
 /* nplans patterns in ONE persistent launch over the resident graph;
  * out[i] per plan, kernel_ms is the launch's total in every entry. */
 int tdfs_count_batch(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plans, int32_t nplans,

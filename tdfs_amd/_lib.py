@@ -107,6 +107,7 @@ _SIGS = {
     "tdfs_graph_info": (C.c_int, [_P, _I64P]),
     "tdfs_graph_order": (C.c_int, [_P, _I32P]),
     "tdfs_count": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int64, C.c_int64, C.POINTER(Result)]),
+    "tdfs_count_noalloc": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int64, C.c_int64, C.POINTER(Result)]),
     "tdfs_count_batch": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int32, C.POINTER(Result)]),
     "tdfs_enumerate": (C.c_int, [_P, _P, C.POINTER(Plan), C.c_int64, C.c_int64, C.POINTER(C.c_int32), C.c_uint64,
                                  C.POINTER(C.c_uint64), C.POINTER(Result)]),

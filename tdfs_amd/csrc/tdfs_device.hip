@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <vector>
 
 #include "tdfs.h"
@@ -1831,6 +1832,9 @@ struct Launch {
   int grid = 0;
 };
 
+static int scratch_setup(int device);
+constexpr int kMaxDevices = 64;
+
 struct tdfs_ctx {
   int device;
   int cus;
@@ -1981,6 +1985,7 @@ extern "C" int tdfs_ctx_create(int32_t device, tdfs_ctx **out) {
   c->steal_blocks_per_cu_stats = std::max(1, sb2);
   for (int w = 0; w < 9; w++) c->steal_blocks_per_cu_w[w] = std::max(1, sbw[w]);
   int rc = launch_init(c->main, device);
+  if (!rc) rc = scratch_setup(device);
   if (rc) {
     launch_free(c->main);
     delete c;
@@ -3115,4 +3120,96 @@ extern "C" int tdfs_enumerate(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_pla
   }
   *written = nw;
   return TDFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tdfs_count_noalloc: tdfs_count for callers that cannot allocate.  Every
+// buffer of the launch (control block, result and statistics words, cursor,
+// plan, a task ring of kScratchRing entries, and the per-wave DFS stacks) is
+// carved from `scratch`, a file-scope device array of kScratchBytes that
+// each GPU holds from module load until the process ends.  Host code never
+// uses the array's name as an address: tdfs_ctx_create looks up its device
+// address once per device (hipGetSymbolAddress, with that device current)
+// and every later use goes through that pointer.  The grid shrinks to the
+// workgroups whose DFS stacks fit the array; a plan whose single workgroup
+// does not fit is refused with TDFS_E_BUDGET.  All calls on one GPU share
+// the array and the context's stream: calls that overlap on one GPU (from
+// two contexts, or two threads) race on it.  The launch's init kernel clears
+// what must start at zero, so the previous call's contents do not matter.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 256ull << 20;
+constexpr unsigned long long kScratchRing = 1ull << 14;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];  // device address of `scratch` on each device
+
+// Set-up, once per device, with that device current (from tdfs_ctx_create).
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(TDFS_E_ARG, "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return TDFS_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return fail(TDFS_E_HIP, "scratch address: %s", hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return TDFS_OK;
+}
+
+extern "C" int tdfs_count_noalloc(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
+                                  int64_t edge_end, tdfs_result *out) {
+  if (!ctx || !g || !plan || !out) return fail(TDFS_E_ARG, "null argument");
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_ptr[ctx->device];
+  }
+  if (!base) return fail(TDFS_E_STATE, "scratch of device %d not set up", ctx->device);
+  // carve the scratch array: 256-byte aligned regions, the stacks last
+  char *at = static_cast<char *>(base);
+  auto take = [&](size_t bytes) {
+    char *p = at;
+    at += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  Launch L;
+  L.device = ctx->device;
+  L.stream = ctx->main.stream;
+  L.ev0 = ctx->main.ev0;
+  L.ev1 = ctx->main.ev1;
+  L.ctrl = reinterpret_cast<Ctrl *>(take(sizeof(Ctrl)));
+  L.counts = reinterpret_cast<unsigned long long *>(take(sizeof(unsigned long long)));
+  L.stats = reinterpret_cast<unsigned long long *>(take(4 * sizeof(unsigned long long)));
+  L.prof = reinterpret_cast<unsigned long long *>(take(8 * sizeof(unsigned long long)));
+  L.cursors = reinterpret_cast<unsigned long long *>(take(16 * sizeof(unsigned long long)));
+  L.plans = reinterpret_cast<DevPlan *>(take(sizeof(DevPlan)));
+  L.plan_cap = 1;
+  L.q_seq = reinterpret_cast<unsigned long long *>(take(kScratchRing * sizeof(unsigned long long)));
+  L.q_data = reinterpret_cast<TaskRec *>(take(kScratchRing * sizeof(TaskRec)));
+  L.q_cap = kScratchRing;
+  const size_t used = (size_t)(at - static_cast<char *>(base));
+  L.wbuf = reinterpret_cast<int *>(at);
+  L.wbuf_ints = (kScratchBytes - used) / sizeof(int);
+  L.wbuf_external = true;
+  // host sides of the plan upload and result copies: this call's own frame
+  DevPlan h_plan;
+  unsigned long long h_counts[1], h_stats[4], h_prof[8];
+  Ctrl h_ctrl;
+  L.h_plans = &h_plan;
+  L.h_counts = h_counts;
+  L.h_stats = h_stats;
+  L.h_prof = h_prof;
+  L.h_ctrl = &h_ctrl;
+  const bool stats = ctx->stats != 0;
+  KParams P;
+  int rc = launch_prepare(ctx, L, g, plan, 1, edge_begin, edge_end, stats, 0, P, nullptr, true);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(L.ev0, L.stream));
+  rc = launch_enqueue(L, P, stats);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(L.ev1, L.stream));
+  HIP_TRY(hipStreamSynchronize(L.stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, L.ev0, L.ev1));
+  return launch_collect(L, plan, 1, ms, out, stats ? ctx->last_prof : nullptr);
 }
