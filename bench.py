@@ -149,6 +149,11 @@ def device_sync() -> None:
         raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
 
 
+def log(*a) -> None:
+    """Progress on stderr (one line per phase or step)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}]", *a, file=sys.stderr, flush=True)
+
+
 def measure(dev, dg, pats, steps: int, warmup: int, plain_steps: int) -> dict:
     """Exact counts + algorithmic bytes (statistics launch), then timed
     batched (hipGraph replay) steps and plain (one launch per pattern) steps."""
@@ -157,7 +162,9 @@ def measure(dev, dg, pats, steps: int, warmup: int, plain_steps: int) -> dict:
     names = list(pats)
     plans = [T.plan_pattern(t, lab) for t, lab in pats.values()]
     dev.set_stats(True)
+    t0 = time.perf_counter()
     st = dev.count_batch(dg, plans)
+    log(f"statistics launch {time.perf_counter() - t0:.2f} s")
     prof = dev.last_profile()
     dev.set_stats(False)
     counts = [int(r.count) for r in st]
@@ -178,22 +185,25 @@ def measure(dev, dg, pats, steps: int, warmup: int, plain_steps: int) -> dict:
             ms += r.kernel_ms
         return ms
 
-    for _ in range(warmup):
-        batched()
+    for i in range(warmup):
+        log(f"warm-up step {i}: {batched():.1f} ms")
     kb, kp = [], []
     device_sync()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         kb.append(batched())
+        if i % 5 == 4 or i == steps - 1:
+            log(f"timed step {i}: {kb[-1]:.1f} ms")
     device_sync()
     t_b = (time.perf_counter() - t0) / steps
     t_p = float("nan")
     if plain_steps > 0:
-        plain()  # warm-up
+        log(f"plain warm-up step: {plain():.1f} ms")
         device_sync()
         t0 = time.perf_counter()
-        for _ in range(plain_steps):
+        for i in range(plain_steps):
             kp.append(plain())
+            log(f"plain step {i}: {kp[-1]:.1f} ms")
         device_sync()
         t_p = (time.perf_counter() - t0) / plain_steps
     batch.close()
@@ -217,6 +227,7 @@ def per_pattern(dev, dg, pats) -> list[dict]:
         plan = T.plan_pattern(text, lab)
         dev.set_stats(False)
         r = dev.count(dg, plan)
+        log(name, f"{r.kernel_ms:.1f} ms")
         dev.set_stats(True)
         dev.count(dg, plan)
         prof = dev.last_profile()
@@ -239,6 +250,7 @@ def serve_leg(queries: int, rates, time_limit_ms: int) -> dict:
 
     d = json.loads((ROOT / "tests" / "golden" / "c5_queries.json").read_text())
     qs = d["queries"][:queries]
+    log("c5 serving leg")
     g = W.graph("c5")
     dev = T.Device(0)
     dg = dev.upload(g)
@@ -320,6 +332,7 @@ def main() -> int:
         del src, dst
 
     g, pats, desc, extra_cfg = load_workload(args.workload)
+    log(f"{args.workload}: graph n={g.n} nnz={g.nnz}, {len(pats)} patterns")
     dev = T.Device(0)
     if tuned:
         dev.set_tuning(**tuned)
@@ -368,6 +381,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline")
         threads = args.cpu_threads or min(16, cpu_info()["usable"])
         cpu = cpu_baseline(g, perm, pats, threads)
 

@@ -107,9 +107,12 @@ typedef struct tdfs_tuning {
   int32_t merge_min;         /* > 0: an extension with two input lists that both hold at least this many
                                 candidates (neither more than 8x the other) intersects them by merge path
                                 (co-rank split of the merge into 8 steps per lane, tiles staged in LDS)
-                                instead of probing the shorter list (default 0: never).  A kernel variant
-                                of its own (4 or 5 waves per SIMD settings other than 5 run the default
-                                kernel's occupancy); it excludes stealing */
+                                instead of probing the shorter list (default 0: never).  Compiled as
+                                kernel variants of their own (default occupancy, and 5 waves per SIMD);
+                                it excludes stealing */
+  int32_t leaf_memo;         /* 1 (default): when no input of the leaf C(k-1) involves M(k-2), the leaf's
+                                intersection over M(0..k-3) is computed once and each sibling M(k-2) only
+                                subtracts itself or applies its bound (one search); 2: off (testing) */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

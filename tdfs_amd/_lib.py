@@ -56,6 +56,7 @@ class Tuning(C.Structure):
         ("idle_deadline_ms", C.c_int32),
         ("steal", C.c_int32),
         ("merge_min", C.c_int32),
+        ("leaf_memo", C.c_int32),
     ]
 
 

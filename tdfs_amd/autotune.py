@@ -6,7 +6,8 @@ occupancy (blocks_per_cu: waves resident per CU, i.e. how many DFS stacks
 share a CU's registers and LDS), level-0 chunk (chunk_edges: edges per cursor
 grab), donation threshold (timeout_us), donated piece size (split_chunk) and
 leaf batch size (leaf_pair_cap), the smallest level that gets a hoisted
-intersection (hoist_min) and the smallest degree given a neighbour bitmap
+intersection (hoist_min), intra-workgroup stealing (steal) and the smallest
+degree given a neighbour bitmap
 (hub_min_degree, read at upload: the graph is uploaded again) --
 timing the hipGraph-replayed batch (median of `reps` replays per point; every
 replay's counts must equal the statistics launch).  The winner per workload is
@@ -32,6 +33,7 @@ SPACE = {
     "leaf_pair_cap": [1024, 4096, 16384],
     "hub_min_degree": [64, 128, 512],
     "hoist_min": [1, 8, 32],
+    "steal": [1, 2],
 }
 # knobs read when the graph is uploaded: a new value means a new upload
 UPLOAD_KNOBS = ("hub_min_degree",)
@@ -77,7 +79,7 @@ def time_point(dev, up: Uploads, plans, counts, knobs: dict, reps: int) -> float
 
 
 DEFAULT = {"waves_per_simd": 0, "blocks_per_cu": 0, "chunk_edges": 64, "timeout_us": 200, "split_chunk": 1,
-           "leaf_pair_cap": 4096, "hub_min_degree": 256, "hoist_min": 2}
+           "leaf_pair_cap": 4096, "hub_min_degree": 256, "hoist_min": 2, "steal": 2}
 
 
 def tune(dev, g, plans, reps: int = 3, rounds: int = 2, log=print, start: dict | None = None,

@@ -84,11 +84,16 @@ struct DevPlan {
   // level-0 directed edges of this plan: [e_lo, e_lo + e_span) (per-plan
   // cursors; the interleaved cursor uses the launch's common range)
   long long e_lo, e_span;
+  // [0] with result reuse, [1] without: 1 when no input of the leaf C(k-1)
+  // involves M(k-2), so the leaf is counted for all its siblings from one
+  // intersection (leaf_memo)
+  int leaf_free[2];
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
-// (kHoisted), or H(u) itself (kBuildH).
-enum { kFull = 0, kHoisted = 1, kBuildH = 2 };
+// (kHoisted), or H(u) itself (kBuildH); kMemo: the leaf's candidates from the
+// plan's inputs filtered against M(0..u-2) only (leaf_memo).
+enum { kFull = 0, kHoisted = 1, kBuildH = 2, kMemo = 3 };
 
 // A donated unit of work: prefix M(0..p-1) and the iteration range [a, b) of
 // C(p).  fresh = 1 marks level-0 donations whose first extension was never
@@ -165,6 +170,7 @@ struct KParams {
   int donate_levels;
   int reuse;
   int leaf_batch;
+  int leaf_memo;             // 1: leaves that do not involve M(k-2) are counted for all siblings at once
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
@@ -586,7 +592,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
     }
   }
   const int *pp = S.ulp[pidx];
-  const bool leaf = (u == pl->k - 1) && mode != kBuildH;
+  const bool leaf = (u == pl->k - 1) && mode != kBuildH && mode != kMemo;
   const bool labeled = pl->labeled != 0;
   if (STATS && count_stats) {
     if (labeled) bytes += (unsigned)minfull;
@@ -595,7 +601,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   // every probe element carries l(u): segments, C(dep) (same label as u) or H(u)
   const bool label_known = !labeled || seg >= 0 || dep >= 0 || mode == kHoisted;
   // filter bounds; H(u) is filtered only against M(0..hfix(u))
-  const int jlim = mode == kBuildH ? pl->hfix[ri][u] + 1 : u;
+  const int jlim = mode == kBuildH ? pl->hfix[ri][u] + 1 : (mode == kMemo ? u - 1 : u);
   int lo = -1, hi = INT_MAX;
   int nev[6];
   int nemask = 0;
@@ -1025,6 +1031,136 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   return found;
 }
 
+// Leaf counts of the owners [i0, i1) of stack level L = k-3 (L >= 1) of a
+// plan whose leaf inputs do not involve the owner's vertex M(k-2) = x
+// (leaf_free).  Then C(k-1) = I minus what x itself excludes, where I is the
+// intersection of the leaf's inputs over the window, labels and injectivity
+// of M(0..k-3) -- the same for every owner.  I is computed once per call:
+// as a window of one input list less the path vertices found in it (one
+// input whose elements all carry the leaf's label), or materialised into the
+// unused H(k-1) slot by extend (kMemo).  Each owner then costs one binary
+// search: elements of I above (GT) or below (LT) x, or I less x itself when
+// x falls under the leaf's injectivity (NE) and lies in I.  Same counts and
+// statistics as leaf_batch.
+template <bool STATS, bool MERGE>
+__device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S,
+                                                        int L, int i0, int i1, const int *lbase, int *hbase,
+                                                        unsigned long long &balg, unsigned long long &next) {
+  const int lane = lane_id();
+  const int u = pl->k - 1;
+  const int ju = u - 1;  // the owner's vertex
+  const bool labeled = pl->labeled != 0;
+  const int ri = P.reuse ? 0 : 1;
+  const int dep = P.reuse ? pl->dep[u] : -1;
+  const bool hoisted = S.hv[u] != 0;  // then C(k-1) = H(k-1) (the owner is no input)
+  const int seg = (labeled && P.colidx_lab) ? pl->label[u] : -1;
+  const int owners = i1 - i0;
+  if (STATS) {
+    // per owner, the merge-model bytes of the reference's inputs (all fixed by the path)
+    unsigned long long bytes = 0;
+    int minfull = INT_MAX;
+    if (dep >= 0) {
+      const int l = S.len[dep - 1];
+      bytes += 4ull * (unsigned)l;
+      minfull = l;
+      for (int i = 0; i < pl->num_bn_sh[u]; i++) {
+        const int d = P.degv[S.M[pl->bn_sh[u][i]]];
+        bytes += 4ull * (unsigned)d + 16;
+        minfull = min(minfull, d);
+      }
+    } else {
+      for (int i = 0; i < pl->num_bn[u]; i++) {
+        const int d = P.degv[S.M[pl->bn[u][i]]];
+        bytes += 4ull * (unsigned)d + 16;
+        minfull = min(minfull, d);
+      }
+    }
+    if (labeled) bytes += (unsigned)minfull;
+    balg += bytes * (unsigned long long)owners;
+    next += (unsigned long long)owners;
+  }
+  // the leaf's relation to the owner
+  const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
+  const int same = (labeled && seg >= 0) ? pl->same_label_mask[u] : 0x7f;
+  const bool c_gt = !labeled && ((gt >> ju) & 1);
+  const bool c_lt = !labeled && ((lt >> ju) & 1);
+  const bool c_ne = !c_gt && !c_lt && ((same >> ju) & 1);
+  // I as a sorted list base[0, len) less `nex` excluded path vertices ex[] that occur in it
+  const int *base;
+  int len;
+  int ex[6];    // path vertices excluded from I (-1: none), at fixed positions
+  int nex = 0;
+  int nin = 0;  // inputs of the leaf
+  int single_len = 0;
+  const int *single = nullptr;
+  if (hoisted) {
+    single = hbase + (long long)(u - 3) * P.dstride;
+    single_len = S.hl[u];
+    nin = 1;
+  } else if (dep >= 0) {
+    single = lbase + (long long)(dep - 2) * P.dstride;
+    single_len = S.len[dep - 1];
+    nin = 1 + pl->num_bn_sh[u];
+  } else {
+    nin = pl->num_bn[u];
+    if (nin == 1) single = adj_list(P, S.M[pl->bn[u][0]], seg, single_len);
+  }
+  const bool label_known = !labeled || seg >= 0 || dep >= 0 || hoisted;
+  int lo = -1, hi = INT_MAX;
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    ex[j] = -1;
+    if (j < ju) {
+      const int m = S.M[j];
+      if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
+      else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
+    }
+  }
+  if (nin == 1 && label_known) {
+    int start = 0, stop = single_len;
+    if (lo >= 0) start = lower_bound_i(single, single_len, lo + 1);
+    if (hi != INT_MAX) stop = lower_bound_i(single, single_len, hi);
+    if (stop < start) stop = start;
+    base = single + start;
+    len = stop - start;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      if (j < ju && ((same >> j) & 1) && !(!labeled && (((gt | lt) >> j) & 1))) {
+        const int v = S.M[j];
+        if (v > lo && v < hi && contains(base, len, v)) {
+          ex[j] = v;
+          nex++;
+        }
+      }
+    }
+  } else {
+    int *buf = hbase + (long long)(u - 3) * P.dstride;  // H(k-1) is not built: its slot is free
+    unsigned long long db = 0, dn = 0;
+    len = (int)extend<false, MERGE>(P, pl, S, u, buf, lbase, hbase, kMemo, false, db, dn);
+    base = buf;
+  }
+  int nI = len - nex;
+  unsigned long long found = 0;
+  for (int o0 = i0; o0 < i1; o0 += kWave) {
+    const int idx = o0 + lane;
+    if (idx < i1) {
+      const int x = lbase[(long long)(L - 1) * P.dstride + idx];
+      int c = nI;
+      if (c_gt || c_lt) {
+        const int pos = lower_bound_i(base, len, c_gt ? x + 1 : x);  // elements below the bound
+        c = c_gt ? len - pos : pos;
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+          if (ex[j] >= 0 && (c_gt ? ex[j] > x : ex[j] < x)) c--;
+      } else if (c_ne && x > lo && x < hi) {
+        if (contains(base, len, x)) c--;  // x itself is no candidate (path vertices are never owners)
+      }
+      found += (unsigned)c;
+    }
+  }
+  return wave_sum(found);
+}
+
 // Level-0 filter over directed edges [e0, e1) (src/gpu_match.cu:526-558).
 // "exists v in N(c) with deg(v) >= dP(2)" is maxnbr[c] >= dP(2).
 template <bool STATS>
@@ -1297,6 +1433,16 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
           if (n > 0) continue;  // re-read this level: its end has moved
         }
       }
+    }
+    if (level + 2 == k - 1 && level > 0 && P.leaf_memo && P.leaf_batch && !P.emit_n &&
+        pl->leaf_free[P.reuse ? 0 : 1]) {
+      int n = 0;
+      const int it = level_claim(S, level, 1024, n, shared);
+      if (n == 0) continue;
+      const long long tp = STATS ? clock64() : 0;
+      cnt += leaf_memo<STATS, MERGE>(P, pl, S, level, it, it + n, lbase, hbase, balg, next);
+      if (STATS) prof[2] += clock64() - tp;
+      continue;
     }
     if (level + 2 == k - 1 && P.leaf_batch) {
       int n = 0;
@@ -1876,6 +2022,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->hoist_min = 2;
   t->idle_deadline_ms = 60000;
   t->steal = 2;
+  t->leaf_memo = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -1903,6 +2050,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->hoist_min == 0) t->hoist_min = d.hoist_min;
   if (t->idle_deadline_ms == 0) t->idle_deadline_ms = d.idle_deadline_ms;
   if (t->steal != 1) t->steal = 2;
+  if (t->leaf_memo != 2) t->leaf_memo = 1;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -2317,6 +2465,18 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
     d.num_bn_sh[u] = d.dep[u] >= 0 ? p.num_bn_sh[u] : 0;
     for (int i = 0; i < d.num_bn_sh[u]; i++) d.bn_sh[u][i] = p.bn_sh[u][i];
   }
+  // leaf_free: no input of C(k-1) involves M(k-2)
+  for (int ri = 0; ri < 2; ri++) {
+    const int u = p.k - 1;
+    d.leaf_free[ri] = 0;
+    if (p.k < 4) continue;
+    const int dep = ri == 0 ? d.dep[u] : -1;
+    const int n_in = dep >= 0 ? d.num_bn_sh[u] : d.num_bn[u];
+    const int *in = dep >= 0 ? d.bn_sh[u] : d.bn[u];
+    bool free_ = true;
+    for (int i = 0; i < n_in; i++) free_ = free_ && in[i] != u - 1;
+    d.leaf_free[ri] = free_ ? 1 : 0;
+  }
   // H(u): the inputs of C(u) other than N(M(u-1)); worth a list of its own
   // when at least two of them remain to intersect
   for (int ri = 0; ri < 2; ri++) {
@@ -2525,6 +2685,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.donate_levels = t.donate_levels;
   P.reuse = t.reuse == 1 ? 1 : 0;
   P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
+  P.leaf_memo = t.leaf_memo == 1 && P.leaf_batch ? 1 : 0;
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
   P.leaf_pair_cap = t.leaf_pair_cap;
@@ -3088,6 +3249,7 @@ extern "C" int tdfs_enumerate(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_pla
   P.emit_cap = cap;
   P.emit_n = d_n;
   P.leaf_batch = 0;  // leaves are walked one parent at a time to write their rows
+  P.leaf_memo = 0;
   hipError_t e = hipMemsetAsync(d_n, 0, sizeof(unsigned long long), L.stream);
   if (e == hipSuccess) e = hipEventRecord(L.ev0, L.stream);
   if (e != hipSuccess) {

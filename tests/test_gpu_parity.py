@@ -187,6 +187,9 @@ def test_scheduling_modes_are_exact(dev):
         dict(merge_min=8, waves_per_simd=5),
         dict(merge_min=1, hoist_min=-1, leaf_batch=2),
         dict(merge_min=4, timeout_us=1, donate_levels=5),
+        dict(leaf_memo=2),
+        dict(hoist_min=1, timeout_us=1, donate_levels=5, steal=1),
+        dict(leaf_memo=2, reuse=2),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
