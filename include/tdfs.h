@@ -113,6 +113,9 @@ typedef struct tdfs_tuning {
   int32_t leaf_memo;         /* 1 (default): when no input of the leaf C(k-1) involves M(k-2), the leaf's
                                 intersection over M(0..k-3) is computed once and each sibling M(k-2) only
                                 subtracts itself or applies its bound (one search); 2: off (testing) */
+  int32_t leaf_hash;         /* 1 (default): a leaf whose inputs are one list shared by all siblings and
+                                the sibling's own list hashes the shared one in LDS (at most 512 elements
+                                in the window) and probes each sibling's list against it; 2: off */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

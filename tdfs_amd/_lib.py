@@ -57,6 +57,7 @@ class Tuning(C.Structure):
         ("steal", C.c_int32),
         ("merge_min", C.c_int32),
         ("leaf_memo", C.c_int32),
+        ("leaf_hash", C.c_int32),
     ]
 
 

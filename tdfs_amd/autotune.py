@@ -128,7 +128,7 @@ def main() -> int:
     import bench
     import tdfs_amd as T
 
-    g, pats, desc = bench.load_workload(args.workload)
+    g, pats, desc, _ = bench.load_workload(args.workload)
     dev = T.Device(0)
     plans = [T.plan_pattern(t, lab) for t, lab in pats.values()]
     out = Path(args.out)

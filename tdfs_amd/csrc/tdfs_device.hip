@@ -56,6 +56,9 @@ constexpr int kBlock = kWavesPerBlock * kWave;
 constexpr long long kSpinMax = 1ll << 24;  // bound on waits for a slot handshake
 constexpr int kMaxSegLabels = 64;          // label segments are built for at most this many labels
 constexpr int kTile = 512;                 // merge-path diagonals per round (8 per lane)
+constexpr int kHashLog = 10;               // leaf hash: 1,024 LDS slots per wave
+constexpr int kHash = 1 << kHashLog;
+constexpr int kHashMax = kHash / 2;        // shared lists of at most this many elements are hashed
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
 struct DevPlan {
@@ -171,6 +174,7 @@ struct KParams {
   int reuse;
   int leaf_batch;
   int leaf_memo;             // 1: leaves that do not involve M(k-2) are counted for all siblings at once
+  int leaf_hash;             // 1: a short shared leaf input is hashed in LDS for membership tests
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
@@ -207,6 +211,13 @@ struct WaveLds {
   const unsigned *uhb[8];
   int uln[8];
   int *tile;  // merge-path variant: this wave's 2 (kTile + 4) LDS ints (the next kTile + 1 of both lists)
+  // leaf hash: this wave's kHash LDS slots (-1 empty) holding the leaf's
+  // shared input list, valid while `epoch` (bumped by every level_set) equals
+  // hash_epoch and the list is the same one
+  int *htab;
+  unsigned epoch, hash_epoch;
+  const int *hash_list;
+  int hash_lo, hash_hi;
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -248,6 +259,7 @@ __device__ __forceinline__ void level_set(WaveLds &S, int l, int it, int end, bo
   if (lane_id() == 0) {
     const unsigned long long w = __hip_atomic_load(&S.ie[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_store(&S.ie[l], ie_make(ie_gen(w) + 1, it, end), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    S.epoch += 1;  // the path changed: a hashed leaf list is stale
   }
 }
 
@@ -706,6 +718,18 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+// Leaf hash (open addressing, linear probing, multiplicative hash).
+__device__ __forceinline__ unsigned hash_slot(int y) { return ((unsigned)y * 2654435761u) >> (32 - kHashLog); }
+__device__ __forceinline__ bool hash_has(const int *tab, int y) {
+  unsigned h = hash_slot(y);
+  for (;;) {
+    const int v = tab[h];
+    if (v == y) return true;
+    if (v < 0) return false;
+    h = (h + 1) & (kHash - 1);
+  }
+}
+
 // Count-only leaf for elements [i0, i1) (at most 64) of stack level L = k-3:
 // element x fixes M(k-2) (for L = 0 the edge pair fixes M(0), M(1)) and
 // contributes |C(k-1)|.  Lane l owns element i0 + l and builds that
@@ -822,7 +846,55 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < nb; i++) add_adj(pl->bn[u][i]);
   }
   const int nl = nu + (hasA ? 1 : 0) + (hasB ? 1 : 0);
-  if (L > 0 && nu == 1 && hasA && !hasB && S.uln[0] <= 2 * kWave) {
+  // One shared input U and the owner's list: when U's window (bounds of the
+  // path before the owner) holds at most kHashMax elements, U goes into this
+  // wave's LDS hash table once per path (kept while the path is unchanged),
+  // each owner probes its own list, and membership in U is an LDS lookup
+  // instead of a chain of dependent global loads.
+  bool use_hash = false;
+  int hash_n = 0;
+  if (P.leaf_hash && L > 0 && nu == 1 && hasA && !hasB) {
+    const int gt0 = pl->gt_mask[u], lt0 = pl->lt_mask[u];
+    int ulo = -1, uhi = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      if (j < u - 1 && !labeled) {
+        if ((gt0 >> j) & 1) ulo = max(ulo, S.M[j]);
+        else if ((lt0 >> j) & 1) uhi = min(uhi, S.M[j]);
+      }
+    }
+    const int *U = S.ulp[0];
+    const int nU = S.uln[0];
+    const int us = ulo >= 0 ? lower_bound_i(U, nU, ulo + 1) : 0;
+    const int ue = uhi != INT_MAX ? lower_bound_i(U, nU, uhi) : nU;
+    // worth it when most parents' own lists are short next to U (else U is
+    // the better probe: bitmap or binary search in each parent's list)
+    const int nact = __popcll(__ballot(act));
+    const int nshort = __popcll(__ballot(act && vn0 <= 4 * (ue - us)));
+    if (ue - us <= kHashMax && 2 * nshort >= nact) {
+      use_hash = true;
+      hash_n = max(0, ue - us);
+      if (!(S.hash_epoch == S.epoch && S.hash_list == U && S.hash_lo == us && S.hash_hi == ue)) {
+        for (int i = lane; i < kHash; i += kWave) S.htab[i] = -1;
+        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        for (int idx = us + lane; idx < ue; idx += kWave) {
+          const int y = U[idx];
+          unsigned h = hash_slot(y);
+          for (;;) {
+            const int old = atomicCAS(S.htab + h, -1, y);
+            if (old < 0 || old == y) break;
+            h = (h + 1) & (kHash - 1);
+          }
+        }
+        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        S.hash_epoch = S.epoch;
+        S.hash_list = U;
+        S.hash_lo = us;
+        S.hash_hi = ue;
+      }
+    }
+  }
+  if (!use_hash && L > 0 && nu == 1 && hasA && !hasB && S.uln[0] <= 2 * kWave) {
     // One shared input U (C(dep), H(u) or N(M(b)), b <= L) of at most 128
     // elements and the owner's N(M(L+1)).  U is the probe for every owner
     // (set semantics: the probe does not change C(u)): lane j holds U[j mod
@@ -908,6 +980,11 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     pn = vn1;
     pidx = 9;
     pp = vp1;
+  }
+  if (use_hash && pidx != 8 && vn0 <= 4 * hash_n) {  // with U hashed, the owner's list is the cheaper probe
+    pn = vn0;
+    pidx = 8;
+    pp = vp0;
   }
   if (STATS && labeled) bytes += (unsigned)minfull;
   const bool label_check = labeled && seg < 0;
@@ -1011,7 +1088,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < nu; i++) {
       if (i != po && ok) {
         const unsigned *h = S.uhb[i];
-        ok = h ? bit_test(h, y) : contains(S.ulp[i], S.uln[i], y);
+        ok = use_hash ? hash_has(S.htab, y) : (h ? bit_test(h, y) : contains(S.ulp[i], S.uln[i], y));
       }
     }
     if (hasA) {
@@ -1594,6 +1671,7 @@ template <bool STATS, bool STEAL, bool MERGE = false>
 __device__ __forceinline__ void match_body(const KParams &P) {
   __shared__ WaveLds lds[kWavesPerBlock];
   __shared__ int tiles[MERGE ? kWavesPerBlock * 2 * (kTile + 4) : 1];
+  __shared__ int htabs[kWavesPerBlock * kHash];
   const int lane = lane_id();
   const int wib = uni(threadIdx.x / kWave);  // wave-uniform: the workspace pointers stay scalar
   const long long gw = (long long)blockIdx.x * kWavesPerBlock + wib;
@@ -1604,6 +1682,10 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   int *hbase = wb + P.hoff;         // H(3..k-1)
   WaveLds &S = lds[wib];
   S.tile = MERGE ? tiles + wib * 2 * (kTile + 4) : nullptr;
+  S.htab = htabs + wib * kHash;
+  S.epoch = 0;
+  S.hash_epoch = ~0u;
+  S.hash_list = nullptr;
   clear_hoists(S);
   if (lane < 8) S.ie[lane] = 0;
   S.readers = 0;
@@ -2023,6 +2105,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->idle_deadline_ms = 60000;
   t->steal = 2;
   t->leaf_memo = 1;
+  t->leaf_hash = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -2051,6 +2134,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->idle_deadline_ms == 0) t->idle_deadline_ms = d.idle_deadline_ms;
   if (t->steal != 1) t->steal = 2;
   if (t->leaf_memo != 2) t->leaf_memo = 1;
+  if (t->leaf_hash != 2) t->leaf_hash = 1;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -2686,6 +2770,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.reuse = t.reuse == 1 ? 1 : 0;
   P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
   P.leaf_memo = t.leaf_memo == 1 && P.leaf_batch ? 1 : 0;
+  P.leaf_hash = t.leaf_hash == 1 ? 1 : 0;
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
   P.leaf_pair_cap = t.leaf_pair_cap;

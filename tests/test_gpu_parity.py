@@ -190,6 +190,8 @@ def test_scheduling_modes_are_exact(dev):
         dict(leaf_memo=2),
         dict(hoist_min=1, timeout_us=1, donate_levels=5, steal=1),
         dict(leaf_memo=2, reuse=2),
+        dict(leaf_hash=2),
+        dict(leaf_hash=1, leaf_pair_cap=64, timeout_us=1, steal=1),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
