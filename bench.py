@@ -89,7 +89,7 @@ def cpu_info() -> dict:
 
 def cpu_baseline(g, perm, pats, threads: int, budget_s: float = 15.0, budget_1t: float = 8.0) -> dict:
     """The oracle on seeded random windows of level-0 edges of the
-    degree-ordered copy the GPU counts on, 16 windows at a time, until
+    degree-ordered copy the GPU counts on, 4 windows per thread at a time, until
     budget_s of wall time has been spent (threads workers), then the same on
     one thread for budget_1t; matches/s = sampled matches / time spent."""
     from oracle import oracle as O
@@ -104,7 +104,7 @@ def cpu_baseline(g, perm, pats, threads: int, budget_s: float = 15.0, budget_1t:
         order = rng.permutation(max(1, H.nnz // win))
         matches, spent, used = 0, 0.0, 0
         while spent < budget and used < order.shape[0]:
-            starts = np.sort(order[used:used + 16]) * win
+            starts = np.sort(order[used:used + 4 * nthreads]) * win
             used += starts.shape[0]
             t0 = time.perf_counter()
             for p in oplans:
@@ -279,6 +279,41 @@ def serve_leg(queries: int, rates, time_limit_ms: int) -> dict:
             "streams": rows}
 
 
+def reach_leg(limit_ms: int) -> list[dict]:
+    """The patterns the configurations name that the benchmark does not time
+    (c2's 5-cycle, c3's labeled 6-path and 6-cycle), each counted alone over
+    the whole graph with a time limit: the kernel time and exact count when it
+    finishes, else the limit as a lower bound on the time and the matches
+    counted until then as a lower bound on the count."""
+    import tdfs_amd as T
+    from tdfs_amd import workloads as W
+
+    tuned_all = json.loads((ROOT / "profiles" / "autotune.json").read_text())
+    rows = []
+    for wl in ("c2", "c3"):
+        g = W.graph(wl)
+        full = W.patterns(wl, full=True)
+        dev = T.Device(0)
+        dev.set_tuning(**tuned_all.get(wl, {}).get("best", {}))
+        dev.set_tuning(time_limit_ms=limit_ms)
+        dg = dev.upload(g)
+        for name in W.OUT_OF_REACH[wl]:
+            text, lab = full[name]
+            log(f"{wl} {name}: counting with a {limit_ms} ms limit")
+            t0 = time.perf_counter()
+            r = dev.count(dg, T.plan_pattern(text, lab))
+            wall = time.perf_counter() - t0
+            done = not (r.flags & 1)
+            rows.append({"workload": wl, "pattern": name, "labeled": bool(lab), "finished": done,
+                         "kernel_ms": r.kernel_ms, "wall_s": round(wall, 2),
+                         "count" if done else "count_lower_bound": int(r.count),
+                         "time_limit_ms": limit_ms})
+            log(json.dumps(rows[-1]))
+        dg.free()
+        dev.close()
+    return rows
+
+
 def nan_none(x: float):
     return None if x != x else x
 
@@ -304,6 +339,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--reach", type=int, default=0, metavar="MS",
+                    help="instead of the bench line: count the patterns the benchmark does not time (c2 5-cycle, "
+                         "c3 6-path and 6-cycle) alone with a time limit of MS ms, one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only. Across GPUs, T-DFS would shard level-0 edge chunks "
@@ -315,6 +353,10 @@ def main() -> int:
     import tdfs_amd as T
     from tdfs_amd import workloads as W
 
+    if args.reach > 0:
+        for row in reach_leg(args.reach):
+            print(json.dumps(row), flush=True)
+        return 0
     tfile = ROOT / "profiles" / "autotune.json"
     tuned_all = json.loads(tfile.read_text()) if tfile.exists() else {}
     tuned = tuned_all.get(args.workload, {}).get("best", {})
@@ -349,12 +391,14 @@ def main() -> int:
     print(f"{args.workload}: batched {m['t_batched'] * 1e3:.1f} ms per step", file=sys.stderr, flush=True)
     kernel = KERNEL_NAMES.get(int(tuned.get("waves_per_simd", 0)), "tdfs_match_kernel")
 
-    traffic = None
+    traffic, counters = None, None
     trfile = ROOT / "profiles" / "traffic.json"
     if trfile.exists():
         rec = json.loads(trfile.read_text()).get(args.workload, {})
         if rec.get("kernel") == kernel and rec.get("patterns") == len(m["names"]):
             traffic = rec.get("hbm_bytes_per_launch")
+            counters = {k: rec[k] for k in ("l2_hit", "waves_resident", "valu_lane_util") if k in rec}
+            counters["source"] = "rocprofv3 counter passes of this kernel on this workload (profiles/traffic.json)"
 
     legs = {}
     if not args.no_legs:
@@ -417,6 +461,7 @@ def main() -> int:
                      "kernel": kernel,
                      "merge_model_bytes_per_launch": m["alg_bytes"],
                      "merge_model_rate_gbs": m["alg_bytes"] / (kms * 1e-3) / 1e9},
+        "counters": counters,
         "phase_cycles_frac": {k: round(v / tot, 4) for k, v in prof.items() if k not in ("total", "waves")},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -309,8 +309,13 @@ int tdfs_serve_submit_range(tdfs_server *sv, const tdfs_plan *plan, int64_t edge
  * waiting queries.  Each query is reported as soon as its own work has ended,
  * while the rest of its launch may still run (result.kernel_ms is then the
  * host time since its launch started).  Call repeatedly.  A launch that fails
- * on the device reports its queries with TDFS_FAILED; an error return means a
- * launch could not be started and its queries are still queued. */
+ * on the device reports its queries with TDFS_FAILED and frees its slot; an
+ * error return means a launch could not be started and its queries are still
+ * queued, at the head, for a later poll (the *nout entries written before are
+ * valid either way).  Tests of these paths force them from the host: the
+ * environment variables TDFS_SERVE_FAIL_START=n and TDFS_SERVE_FAIL_RUN=n, read
+ * at tdfs_serve_create, make the server's n-th launch fail to start, or report
+ * its run as failed. */
 int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *results, double *latency_s, int32_t cap,
                     int32_t *nout);
 /* out4 = {queries waiting, slots busy, slots, queries running and not yet reported} */

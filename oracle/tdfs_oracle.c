@@ -557,7 +557,11 @@ static int count_prepared(const orc_graph *gp, const orc_plan *p, const int64_t 
     t->pre = pre;
     t->nranges = nranges;
     t->total = pre[nranges];
-    t->chunk = 64;
+    /* small chunks when the union is short next to the thread count (hub
+       edges make single chunks slow: fine chunks keep the threads balanced) */
+    t->chunk = pre[nranges] / ((int64_t)nthreads * 64);
+    if (t->chunk < 1) t->chunk = 1;
+    if (t->chunk > 64) t->chunk = 64;
     t->cursor = &cursor;
     t->mu = &mu;
     if (nthreads > 1) pthread_create(&t->th, NULL, thread_main, t);

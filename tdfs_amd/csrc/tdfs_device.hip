@@ -36,6 +36,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -2999,6 +3000,7 @@ struct ServeSlot {
   unsigned long long *done_dev = nullptr;   // the same memory as the device sees it
   size_t off = 0, bytes = 0;                // stack region of the arena in use
   double t_launch = 0;
+  long long launch_no = 0;                  // server-wide number of the launch in this slot (1-based)
   bool busy = false;
 };
 
@@ -3016,6 +3018,10 @@ struct tdfs_server {
   std::vector<ServeSlot> slots;
   std::deque<ServeQuery> waiting;
   uint64_t next_ticket = 1;
+  long long launches = 0;  // launches started (or attempted)
+  // error-path tests: launch number whose start fails (TDFS_SERVE_FAIL_START)
+  // or whose device run is taken as failed (TDFS_SERVE_FAIL_RUN); 0: none
+  long long fail_start = 0, fail_run = 0;
 };
 
 static double now_s() {
@@ -3064,6 +3070,9 @@ extern "C" int tdfs_serve_create(tdfs_ctx *ctx, const tdfs_graph *g, int32_t slo
     delete sv;
     return fail(TDFS_E_OOM, "serving stack arena of %zu bytes", b);
   }
+  // host-forced failures for tests of the error paths (the kernels run normally)
+  if (const char *s = std::getenv("TDFS_SERVE_FAIL_START")) sv->fail_start = std::atoll(s);
+  if (const char *s = std::getenv("TDFS_SERVE_FAIL_RUN")) sv->fail_run = std::atoll(s);
   tdfs_tuning t = ctx->tune;
   fill_defaults(&t);
   sv->slots.resize(slots);
@@ -3179,8 +3188,10 @@ extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *
   };
   for (auto &sl : sv->slots) {
     if (!sl.busy) continue;
+    // a launch forced to fail reports every query at its end, all as failed
+    const bool forced = sv->fail_run > 0 && sl.launch_no == sv->fail_run;
     // queries whose own work has ended
-    for (size_t i = 0; i < sl.running.size() && n < cap; i++) {
+    for (size_t i = 0; i < sl.running.size() && n < cap && !forced; i++) {
       if (sl.reported[i]) continue;
       if (!(__atomic_load_n(&sl.done_host[2 * i + 1], __ATOMIC_ACQUIRE) & 1ull)) continue;
       tdfs_result r;
@@ -3189,7 +3200,7 @@ extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *
     }
     const hipError_t e = hipEventQuery(sl.L.ev1);
     if (e == hipErrorNotReady) continue;
-    const bool failed = e != hipSuccess || sl.L.h_ctrl->error != 0;
+    const bool failed = e != hipSuccess || sl.L.h_ctrl->error != 0 || forced;
     size_t left = 0;
     for (size_t i = 0; i < sl.running.size(); i++) left += !sl.reported[i];
     if ((int)left > cap - n) continue;  // no room to report the rest of this slot yet
@@ -3203,7 +3214,8 @@ extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *
     }
     if (failed)
       fail(TDFS_E_HIP, "serving launch failed: %s",
-           e != hipSuccess ? hipGetErrorString(e) : "device scheduler error");
+           e != hipSuccess ? hipGetErrorString(e)
+                           : (sl.L.h_ctrl->error != 0 ? "device scheduler error" : "forced (TDFS_SERVE_FAIL_RUN)"));
     sl.running.clear();
     sl.reported.clear();
     sl.busy = false;
@@ -3238,9 +3250,12 @@ extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *
     sl.L.wbuf = reinterpret_cast<int *>(sv->arena + off);
     sl.L.wbuf_ints = bytes / sizeof(int);
     std::memset(sl.done_host, 0, sizeof(unsigned long long) * 2 * take);
+    const long long no = ++sv->launches;
     KParams P;
-    int rc = launch_prepare(sv->ctx, sl.L, sv->g, plans.data(), (int)plans.size(), -1, -1, false, sv->blocks_per_cu,
-                            P, ranges.data(), true);
+    int rc = sv->fail_start > 0 && no == sv->fail_start
+                 ? fail(TDFS_E_OOM, "launch %lld not started (forced: TDFS_SERVE_FAIL_START)", no)
+                 : launch_prepare(sv->ctx, sl.L, sv->g, plans.data(), (int)plans.size(), -1, -1, false,
+                                  sv->blocks_per_cu, P, ranges.data(), true);
     if (!rc) {
       P.pend = sl.L.cursors + 8;
       P.done_host = sl.done_dev;
@@ -3261,6 +3276,7 @@ extern "C" int tdfs_serve_poll(tdfs_server *sv, uint64_t *tickets, tdfs_result *
     sl.off = off;
     sl.bytes = bytes;
     sl.t_launch = now_s();
+    sl.launch_no = no;
     sl.busy = true;
   }
   *nout = n;

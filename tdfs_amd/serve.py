@@ -25,7 +25,17 @@ import time
 import numpy as np
 
 from . import DeviceGraph, Device, Plan, Result, gen_pattern, plan_pattern
-from ._lib import check, i64p, lib
+from ._lib import TdfsError, check, i64p, lib
+
+
+class ServeStartError(TdfsError):
+    """tdfs_serve_poll could not start a launch (its queries stay queued and
+    start on a later poll); `results` holds the queries that the same poll
+    did collect."""
+
+    def __init__(self, code: int, results: list):
+        super().__init__(code, "serve poll")
+        self.results = results
 
 
 class Server:
@@ -49,8 +59,11 @@ class Server:
         res = (Result * self.cap)()
         lat = (C.c_double * self.cap)()
         n = C.c_int32()
-        check(lib().tdfs_serve_poll(self.handle, tickets, res, lat, self.cap, C.byref(n)), "serve poll")
-        return [(int(tickets[i]), res[i], float(lat[i])) for i in range(n.value)]
+        rc = lib().tdfs_serve_poll(self.handle, tickets, res, lat, self.cap, C.byref(n))
+        out = [(int(tickets[i]), res[i], float(lat[i])) for i in range(n.value)]
+        if rc != 0:
+            raise ServeStartError(rc, out)
+        return out
 
     def status(self) -> dict:
         out = np.zeros(4, dtype=np.int64)
@@ -116,12 +129,20 @@ def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progre
     t0 = time.perf_counter()
     nxt = 0
     ticket_of = {}
+    start_errors = 0
     while len(results) < len(plans):
         now = time.perf_counter() - t0
         while nxt < len(plans) and arrivals[nxt] <= now:
             ticket_of[server.submit(plans[nxt])] = nxt
             nxt += 1
-        for tk, r, l in server.poll():
+        try:
+            got = server.poll()
+        except ServeStartError as e:  # a launch could not start: its queries stay queued
+            got = e.results
+            start_errors += 1
+            if start_errors > 100:
+                raise
+        for tk, r, l in got:
             i = ticket_of[tk]
             results[i] = r
             lat[i] = l
@@ -148,6 +169,7 @@ def run_stream(server: Server, plans: list[Plan], rate: float, seed: int, progre
         "mean_ms": float(lats.mean() * 1e3),
         "matches": int(sum(exact)),
         "matches_per_s": sum(exact) / wall,
+        "start_errors": start_errors,
         "counts": counts,
     }
 

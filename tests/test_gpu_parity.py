@@ -495,6 +495,47 @@ def test_serving_mix_equals_alone(dev):
     dev.set_tuning(steal=base.steal, timeout_us=base.timeout_us)
 
 
+def test_serving_error_paths(dev, monkeypatch):
+    """A launch that cannot start leaves its query queued at the head (it runs
+    on a later poll, exactly), and a launch whose run fails reports its query
+    once with TDFS_FAILED and frees the slot; every other query stays exact and
+    no ticket is lost or reported twice.  Both failures are forced from the
+    host (the kernels run normally): launch 2 fails to start, launch 4's run is
+    taken as failed."""
+    from tdfs_amd import serve
+    G = T.CSRGraph.chung_lu(20000, 100000, 2.2, 1500, 63)
+    dg = dev.upload(G)
+    plans = [T.plan_pattern(fixture(i), 0) for i in (1, 2, 3, 7, 1, 2)]
+    alone = [dev.count(dg, p).count for p in plans]
+    monkeypatch.setenv("TDFS_SERVE_FAIL_START", "2")
+    monkeypatch.setenv("TDFS_SERVE_FAIL_RUN", "4")
+    sv = serve.Server(dev, dg, slots=1, blocks_per_cu=1, max_batch=1)
+    ticket_of = {sv.submit(p): i for i, p in enumerate(plans)}
+    got, start_errors = {}, 0
+    t0 = time.time()
+    while len(got) < len(plans) and time.time() - t0 < 60:
+        try:
+            out = sv.poll()
+        except serve.ServeStartError as e:
+            assert e.code == -5 and "forced" in str(e)
+            out = e.results
+            start_errors += 1
+        for tk, r, _ in out:
+            assert tk in ticket_of and tk not in got
+            got[tk] = (int(r.count), int(r.flags))
+        time.sleep(0.001)
+    sv.close()
+    say("serving error paths", start_errors, got)
+    assert start_errors == 1 and len(got) == len(plans)
+    by_query = {ticket_of[tk]: v for tk, v in got.items()}
+    # launches: 1 = q0, 2 = q1 (not started), 3 = q1, 4 = q2 (run failed), 5.. = q3..
+    for i, (c, f) in sorted(by_query.items()):
+        if i == 2:
+            assert f & 2, (i, f)
+        else:
+            assert f == 0 and c == alone[i], (i, c, f, alone[i])
+
+
 def test_serving_refuses_over_budget(dev):
     from tdfs_amd import serve
     G = T.CSRGraph.chung_lu(20000, 100000, 2.1, 2000, 62)
