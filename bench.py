@@ -231,11 +231,13 @@ def per_pattern(dev, dg, pats) -> list[dict]:
         dev.set_stats(True)
         dev.count(dg, plan)
         prof = dev.last_profile()
+        ctr = dev.last_counters()
         dev.set_stats(False)
         tot = max(1, prof["total"])
         rows.append({"pattern": name, "k": int(plan.k), "labeled": bool(plan.labeled), "kernel_ms": r.kernel_ms,
                      "count": int(r.count),
-                     "cycles_frac": {k: round(prof[k] / tot, 3) for k in ("filter", "extend", "leaf", "idle")}})
+                     "cycles_frac": {k: round(prof[k] / tot, 3) for k in ("filter", "extend", "leaf", "idle")},
+                     "cycles": prof, "counters": ctr})
     rows.sort(key=lambda x: -x["kernel_ms"])
     return rows
 
@@ -339,6 +341,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--tune", default="", metavar="K=V,...",
+                    help="tuning fields set over the workload's tuned point (tdfs_tuning names), e.g. leaf_hash=3")
     ap.add_argument("--reach", type=int, default=0, metavar="MS",
                     help="instead of the bench line: count the patterns the benchmark does not time (c2 5-cycle, "
                          "c3 6-path and 6-cycle) alone with a time limit of MS ms, one JSON line each")
@@ -359,7 +363,10 @@ def main() -> int:
         return 0
     tfile = ROOT / "profiles" / "autotune.json"
     tuned_all = json.loads(tfile.read_text()) if tfile.exists() else {}
-    tuned = tuned_all.get(args.workload, {}).get("best", {})
+    tuned = dict(tuned_all.get(args.workload, {}).get("best", {}))
+    for kv in filter(None, args.tune.split(",")):
+        k, v = kv.split("=")
+        tuned[k.strip()] = int(v)
 
     if args.copy_calibration > 0:
         import torch

@@ -114,8 +114,15 @@ typedef struct tdfs_tuning {
                                 intersection over M(0..k-3) is computed once and each sibling M(k-2) only
                                 subtracts itself or applies its bound (one search); 2: off (testing) */
   int32_t leaf_hash;         /* 1 (default): a leaf whose inputs are one list shared by all siblings and
-                                the sibling's own list hashes the shared one in LDS (at most 512 elements
-                                in the window) and probes each sibling's list against it; 2: off */
+                                the sibling's own list holds the shared one in LDS -- as a bitmap when its
+                                window spans at most 32,768 ids densely (a sibling with a neighbour bitmap
+                                then intersects 32 ids per load: word AND + popcount), else as a hash table
+                                when the window holds at most 512 elements -- and tests each sibling's list
+                                against it; 3: hash table only; 2: off */
+  int32_t ext_hash;          /* 1 (default): an extension holds one input other than the probe that has no
+                                neighbour bitmap and at most 512 elements in an LDS hash table, so each
+                                candidate's membership test is an LDS lookup instead of a binary search
+                                in HBM; 2: off */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */
@@ -205,6 +212,12 @@ void tdfs_ctx_destroy(tdfs_ctx *ctx);
  * counts, idle waiting, prefix rebuild of received tasks, donation, total,
  * number of waves}. */
 int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8);
+/* Operation counters of the last statistics launch (summed over waves):
+ * {leaf batches, their parents, their units tested (pairs, bitmap words or
+ * list elements), their set-up cycles, extensions, their probe elements,
+ * their membership tests by binary search (per 64-lane round and input),
+ * memoised-leaf calls}.  No counterpart in the reference (a profiling aid). */
+int tdfs_ctx_last_counters(const tdfs_ctx *ctx, uint64_t *out8);
 /* device properties: out4 = {compute units, wall-clock kHz, max waves resident, reserved} */
 int tdfs_ctx_info(const tdfs_ctx *ctx, int64_t *out4);
 

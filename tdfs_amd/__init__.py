@@ -252,6 +252,14 @@ class Device:
         keys = ["filter", "extend", "leaf", "idle", "rebuild", "donate", "total", "waves"]
         return {k: int(v) for k, v in zip(keys, out)}
 
+    def last_counters(self) -> dict:
+        """Operation counters of the last stats launch, summed over waves."""
+        out = (C.c_uint64 * 8)()
+        check(lib().tdfs_ctx_last_counters(self.handle, out), "last counters")
+        keys = ["leaf_batches", "leaf_owners", "leaf_units", "leaf_setup_cycles", "extensions", "probe_elements",
+                "search_tests", "memo_calls"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def tuning(self) -> Tuning:
         t = Tuning()
         check(lib().tdfs_ctx_get_tuning(self.handle, C.byref(t)), "get tuning")

@@ -58,6 +58,7 @@ class Tuning(C.Structure):
         ("merge_min", C.c_int32),
         ("leaf_memo", C.c_int32),
         ("leaf_hash", C.c_int32),
+        ("ext_hash", C.c_int32),
     ]
 
 
@@ -103,6 +104,7 @@ _SIGS = {
     "tdfs_ctx_set_stats": (C.c_int, [_P, C.c_int32]),
     "tdfs_ctx_info": (C.c_int, [_P, _I64P]),
     "tdfs_ctx_last_profile": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "tdfs_ctx_last_counters": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "tdfs_ctx_destroy": (None, [_P]),
     "tdfs_graph_upload": (C.c_int, [_P, C.c_int32, C.c_int64, _I64P, _I32P, _I32P, C.POINTER(_P)]),
     "tdfs_graph_free": (None, [_P]),

@@ -59,6 +59,7 @@ constexpr int kMaxSegLabels = 64;          // label segments are built for at mo
 constexpr int kTile = 512;                 // merge-path diagonals per round (8 per lane)
 constexpr int kHashLog = 10;               // leaf hash: 1,024 LDS slots per wave
 constexpr int kHash = 1 << kHashLog;
+constexpr int kProf = 16;                  // profile words per launch (8 cycle / wave figures, 8 counters)
 constexpr int kHashMax = kHash / 2;        // shared lists of at most this many elements are hashed
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
@@ -92,6 +93,11 @@ struct DevPlan {
   // involves M(k-2), so the leaf is counted for all its siblings from one
   // intersection (leaf_memo)
   int leaf_free[2];
+  // bit w (2 <= w <= k-3): vertex w is free -- no later vertex has it as a
+  // backward neighbour or shares its label (labeled plans), so no later
+  // candidate set depends on M(w) and every M(w) has the same subtree count
+  int free_mask;
+  int pad2_;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -164,7 +170,8 @@ struct KParams {
   // results: counts[pid]; stats[pid*4 + {0 alg bytes, 1 extensions, 2 donated}]
   unsigned long long *counts;
   unsigned long long *stats;
-  unsigned long long *prof;  // 8 per launch: cycles in filter, extend, leaf, idle, rebuild, donate, total; waves
+  unsigned long long *prof;  // kProf per launch: cycles in filter, extend, leaf, idle, rebuild, donate, total;
+                             // waves; then the operation counters of the statistics kernel (WaveLds::ctr)
   Ctrl *ctrl;
   // tuning
   long long timeout_ticks;  // < 0: never donate
@@ -175,7 +182,9 @@ struct KParams {
   int reuse;
   int leaf_batch;
   int leaf_memo;             // 1: leaves that do not involve M(k-2) are counted for all siblings at once
-  int leaf_hash;             // 1: a short shared leaf input is hashed in LDS for membership tests
+  int leaf_hash;             // bit 0: a short shared leaf input may be hashed in LDS for membership tests;
+                             // bit 1: a dense one may be held as an LDS bitmap
+  int ext_hash;              // 1: an extension hashes one short non-probe input in LDS for membership tests
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
@@ -212,14 +221,37 @@ struct WaveLds {
   const unsigned *uhb[8];
   int uln[8];
   int *tile;  // merge-path variant: this wave's 2 (kTile + 4) LDS ints (the next kTile + 1 of both lists)
-  // leaf hash: this wave's kHash LDS slots (-1 empty) holding the leaf's
-  // shared input list, valid while `epoch` (bumped by every level_set) equals
-  // hash_epoch and the list is the same one
+  // this wave's kHash LDS words holding the window [tab_lo, tab_hi) of the
+  // leaf's shared input list tab_list, as a hash table (tab_kind 1: slots,
+  // -1 empty) or a bitmap (tab_kind 2: bit y - tab_base of word
+  // (y - tab_base) / 32 set for every element y in [tab_base, tab_end), path
+  // vertices cleared), or the memoised leaf's set (tab_kind 3: sorted, tab_lo
+  // elements); tab_kind 4: the path's leaf uses no table, 0: nothing held.
+  // Valid while `epoch` (bumped by every level_set) equals tab_epoch
   int *htab;
-  unsigned epoch, hash_epoch;
-  const int *hash_list;
-  int hash_lo, hash_hi;
+  unsigned epoch, tab_epoch;
+  int tab_kind;
+  const int *tab_list;
+  int tab_lo, tab_hi, tab_base, tab_end;
+  // statistics kernel only (lane 0): leaf batches, their owners, their units
+  // (pairs, words or elements tested), their set-up cycles; extensions, their
+  // probe elements, their membership tests by binary search (per 64-lane
+  // round and input); memoised-leaf calls
+  unsigned long long ctr[8];
+  // free levels: the count when the level's first (measured) iteration began
+  unsigned long long mbase[8];
 };
+
+template <bool STATS>
+__device__ __forceinline__ void count_op(WaveLds &S, int i, unsigned long long v) {
+  if (STATS && (threadIdx.x & (kWave - 1)) == 0) S.ctr[i] += v;
+}
+
+// LDS-typed views of a wave's table (ds_* instructions instead of flat ones)
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) unsigned lds_uint;
+__device__ __forceinline__ lds_int *lds_i(int *p) { return (lds_int *)p; }
+__device__ __forceinline__ lds_uint *lds_u(int *p) { return (lds_uint *)p; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -397,6 +429,57 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int n, int x) {
   return lo;
 }
 
+// The same for a search every lane of the wave makes with the same (a, n,
+// x), all lanes active: each round loads 64 evenly spaced elements of the
+// remaining range at once and keeps the block between the last sample below
+// x and the next, so a search costs about log64(n) + 1 dependent loads
+// instead of log2(n).
+__device__ __forceinline__ int lower_bound_wave(const int *a, int n, int x) {
+  const int lane = lane_id();
+  int lo = 0, len = n;  // the answer lies in [lo, lo + len]
+  while (len > kWave) {
+    const int step = (len + kWave - 1) / kWave;
+    const int p = lo + lane * step;
+    const int c = __popcll(__ballot(p < lo + len && a[p] < x));
+    if (c == 0) return lo;  // a[lo] >= x
+    const int nlo = lo + (c - 1) * step + 1;
+    len = min(lo + c * step, lo + len) - nlo;
+    lo = nlo;
+  }
+  return lo + __popcll(__ballot(lane < len && a[lo + lane] < x));
+}
+
+// Several searches at once: the lanes of each group of G (aligned) search
+// sorted a[0, n) for the key they share (G-ary: each round samples G
+// elements of the group's range); all lanes call, and each gets its group's
+// first index i with a[i] >= key.  64 / G independent searches cost about
+// log_G(n) + 1 dependent loads together.
+template <int G>
+__device__ __forceinline__ int lower_bound_group(const int *a, int n, int key) {
+  const int lane = lane_id();
+  const int sub = lane & (G - 1);
+  const int shift = lane & ~(G - 1);
+  const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  int lo = 0, len = n;  // the group's answer lies in [lo, lo + len]
+  while (__ballot(len > G)) {
+    if (len > G) {
+      const int step = (len + G - 1) / G;
+      const int p = lo + sub * step;
+      const unsigned long long m = __ballot(p < lo + len && a[p] < key);
+      const int c = __popcll((m >> shift) & gmask);
+      if (c == 0) {
+        len = 0;  // a[lo] >= key
+      } else {
+        const int nlo = lo + (c - 1) * step + 1;
+        len = min(lo + c * step, lo + len) - nlo;
+        lo = nlo;
+      }
+    }
+  }
+  const unsigned long long m = __ballot(sub < len && a[lo + sub] < key);
+  return lo + __popcll((m >> shift) & gmask);
+}
+
 // Membership of x in sorted a[0, n): branch-free halving, trip count depends
 // only on n.
 __device__ __forceinline__ bool contains(const int *a, int n, int x) {
@@ -408,6 +491,44 @@ __device__ __forceinline__ bool contains(const int *a, int n, int x) {
     n -= half;
   }
   return *base == x;
+}
+
+// LDS hash tables (open addressing, linear probing, multiplicative hash).
+// A table of 2^hlog slots holds at most half as many elements.
+__device__ __forceinline__ unsigned hash_slot(int y, int hlog = kHashLog) {
+  return ((unsigned)y * 2654435761u) >> (32 - hlog);
+}
+__device__ __forceinline__ bool hash_has(const lds_int *tab, int y, int hlog = kHashLog) {
+  unsigned h = hash_slot(y, hlog);
+  const unsigned mask = (1u << hlog) - 1u;
+  for (;;) {
+    const int v = tab[h];
+    if (v == y) return true;
+    if (v < 0) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+// Fills this wave's LDS table with a hash of a[0, n) (n <= 2^(hlog-1)).
+// All lanes call it; the table is complete when it returns.
+__device__ __forceinline__ void hash_build(lds_int *tab, const int *a, int n, int hlog) {
+  const int lane = lane_id();
+  const int slots = 1 << hlog;
+  for (int i = lane; i < slots; i += kWave) tab[i] = -1;
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  const unsigned mask = (unsigned)slots - 1u;
+  for (int idx = lane; idx < n; idx += kWave) {
+    const int y = a[idx];
+    unsigned h = hash_slot(y, hlog);
+    for (;;) {
+      int old = -1;
+      __hip_atomic_compare_exchange_strong(tab + h, &old, y, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (old < 0 || old == y) break;
+      h = (h + 1) & mask;
+    }
+  }
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 
 // Membership through a hub's dense neighbour bitmap: one load.
@@ -536,6 +657,7 @@ template <bool STATS, bool MERGE = false>
 __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u,
                            int *__restrict__ out, const int *lbase, const int *hbase, int mode, bool count_stats,
                            unsigned long long &balg, unsigned long long &next) {
+  count_op<STATS>(S, 4, 1);
   const int lane = lane_id();
   const int ri = P.reuse ? 0 : 1;
   const int dep = P.reuse ? pl->dep[u] : -1;
@@ -636,8 +758,8 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
     }
   }
   int start = 0, stop = pn;
-  if (lo >= 0) start = lower_bound_i(pp, pn, lo + 1);
-  if (hi != INT_MAX) stop = lower_bound_i(pp, pn, hi);
+  if (lo >= 0) start = lower_bound_wave(pp, pn, lo + 1);
+  if (hi != INT_MAX) stop = lower_bound_wave(pp, pn, hi);
   unsigned n = 0;
   // merge path: two inputs, both long, neither more than 8x the other
   int mo = -1, ms = 0, me = 0;
@@ -645,8 +767,8 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
     const int oi = 1 - pidx;
     const int *ob = S.ulp[oi];
     const int on = S.uln[oi];
-    ms = lo >= 0 ? lower_bound_i(ob, on, lo + 1) : 0;
-    me = hi != INT_MAX ? lower_bound_i(ob, on, hi) : on;
+    ms = lo >= 0 ? lower_bound_wave(ob, on, lo + 1) : 0;
+    me = hi != INT_MAX ? lower_bound_wave(ob, on, hi) : on;
     if (me - ms >= P.merge_min && me - ms <= 8 * (stop - start)) mo = oi;
   }
   if (start < stop && mo >= 0) {
@@ -669,13 +791,40 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         if ((nemask >> j) & 1) {
           const int v = nev[j];
           if (v > lo && v < hi) {
-            const int pos = lower_bound_i(pp + start, stop - start, v);
+            const int pos = lower_bound_wave(pp + start, stop - start, v);
             if (pos < stop - start && pp[start + pos] == v) n--;
           }
         }
       }
     } else {
       const int lab = pl->label[u];
+      // the longest other input without a neighbour bitmap, when it has at
+      // most kHashMax elements, goes into this wave's LDS hash table (sized to
+      // it): one coalesced pass over it instead of a binary search per probe
+      int hidx = -1, hlog = 0;
+      if (P.ext_hash) {
+        int best = 7;  // a search of at most 3 loads is kept
+        for (int i = 0; i < nl; i++) {
+          const int l = S.uln[i];
+          if (i != pidx && !S.uhb[i] && l <= kHashMax && l > best) {
+            best = l;
+            hidx = i;
+          }
+        }
+        if (hidx >= 0) {
+          hlog = 6;
+          while ((1 << (hlog - 1)) < best) hlog++;
+          hash_build(lds_i(S.htab), S.ulp[hidx], best, hlog);
+          if (lane == 0) S.tab_kind = 0;  // a leaf's table held there is gone
+        }
+      }
+      const lds_int *tab = lds_i(S.htab);
+      if (STATS) {
+        int ns = 0;
+        for (int i = 0; i < nl; i++) ns += (i != pidx && i != hidx && !S.uhb[i]) ? 1 : 0;
+        count_op<STATS>(S, 5, (unsigned)(stop - start));
+        count_op<STATS>(S, 6, (unsigned long long)ns * (unsigned)((stop - start + kWave - 1) / kWave));
+      }
       for (int base = start; base < stop; base += kWave) {
         const int idx = base + lane;
         bool ok = idx < stop;
@@ -690,7 +839,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
           for (int i = 0; i < nl; i++) {
             if (i != pidx && ok) {
               const unsigned *h = S.uhb[i];
-              ok = h ? bit_test(h, x) : contains(S.ulp[i], S.uln[i], x);
+              ok = i == hidx ? hash_has(tab, x, hlog) : (h ? bit_test(h, x) : contains(S.ulp[i], S.uln[i], x));
             }
           }
         }
@@ -719,18 +868,6 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-// Leaf hash (open addressing, linear probing, multiplicative hash).
-__device__ __forceinline__ unsigned hash_slot(int y) { return ((unsigned)y * 2654435761u) >> (32 - kHashLog); }
-__device__ __forceinline__ bool hash_has(const int *tab, int y) {
-  unsigned h = hash_slot(y);
-  for (;;) {
-    const int v = tab[h];
-    if (v == y) return true;
-    if (v < 0) return false;
-    h = (h + 1) & (kHash - 1);
-  }
-}
-
 // Count-only leaf for elements [i0, i1) (at most 64) of stack level L = k-3:
 // element x fixes M(k-2) (for L = 0 the edge pair fixes M(0), M(1)) and
 // contributes |C(k-1)|.  Lane l owns element i0 + l and builds that
@@ -747,6 +884,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
                                          int i0, int i1, const int *L0r, const int *L0c, const int *lbase,
                                          const int *hbase, unsigned long long &balg, unsigned long long &next,
                                          int &taken) {
+  const long long t_setup = STATS ? clock64() : 0;
   const int lane = lane_id();
   const int k = pl->k;
   const int u = k - 1;
@@ -847,53 +985,180 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < nb; i++) add_adj(pl->bn[u][i]);
   }
   const int nl = nu + (hasA ? 1 : 0) + (hasB ? 1 : 0);
-  // One shared input U and the owner's list: when U's window (bounds of the
-  // path before the owner) holds at most kHashMax elements, U goes into this
-  // wave's LDS hash table once per path (kept while the path is unchanged),
-  // each owner probes its own list, and membership in U is an LDS lookup
-  // instead of a chain of dependent global loads.
-  bool use_hash = false;
-  int hash_n = 0;
+  // One shared input U and the owner's list.  The window of U (bounds of the
+  // path before the owner) goes into this wave's LDS table once per path
+  // (kept while the path is unchanged): as a bitmap when its elements span at
+  // most 32 kHash ids (the dense core of a degree-ordered power-law graph
+  // sits in one such span), else as a hash table when it holds at most
+  // kHashMax elements.  Membership in U is then an LDS lookup instead of a
+  // chain of dependent global loads, and with the bitmap an owner that has a
+  // neighbour bitmap intersects 32 ids per load (word AND + popcount).
+  // The decision, the window and the table depend on the path only: they
+  // are made by the path's first batch and kept in LDS for the others
+  // (tab_kind 4: no table for this path).
+  bool use_hash = false, use_bits = false;
+  int hash_n = 0, bbase = 0, bend = 0;
   if (P.leaf_hash && L > 0 && nu == 1 && hasA && !hasB) {
-    const int gt0 = pl->gt_mask[u], lt0 = pl->lt_mask[u];
-    int ulo = -1, uhi = INT_MAX;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-      if (j < u - 1 && !labeled) {
-        if ((gt0 >> j) & 1) ulo = max(ulo, S.M[j]);
-        else if ((lt0 >> j) & 1) uhi = min(uhi, S.M[j]);
-      }
-    }
     const int *U = S.ulp[0];
-    const int nU = S.uln[0];
-    const int us = ulo >= 0 ? lower_bound_i(U, nU, ulo + 1) : 0;
-    const int ue = uhi != INT_MAX ? lower_bound_i(U, nU, uhi) : nU;
-    // worth it when most parents' own lists are short next to U (else U is
-    // the better probe: bitmap or binary search in each parent's list)
-    const int nact = __popcll(__ballot(act));
-    const int nshort = __popcll(__ballot(act && vn0 <= 4 * (ue - us)));
-    if (ue - us <= kHashMax && 2 * nshort >= nact) {
-      use_hash = true;
-      hash_n = max(0, ue - us);
-      if (!(S.hash_epoch == S.epoch && S.hash_list == U && S.hash_lo == us && S.hash_hi == ue)) {
-        for (int i = lane; i < kHash; i += kWave) S.htab[i] = -1;
-        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        for (int idx = us + lane; idx < ue; idx += kWave) {
-          const int y = U[idx];
-          unsigned h = hash_slot(y);
-          for (;;) {
-            const int old = atomicCAS(S.htab + h, -1, y);
-            if (old < 0 || old == y) break;
-            h = (h + 1) & (kHash - 1);
-          }
+    int kind = 0;
+    if (S.tab_epoch == S.epoch && S.tab_list == U) kind = S.tab_kind;
+    if (kind == 1 || kind == 2 || kind == 4) {
+      use_hash = kind == 1;
+      use_bits = kind == 2;
+      hash_n = S.tab_hi - S.tab_lo;
+      bbase = S.tab_base;
+      bend = S.tab_end;
+    } else {
+      const int gt0 = pl->gt_mask[u], lt0 = pl->lt_mask[u];
+      int ulo = -1, uhi = INT_MAX;
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        if (j < u - 1 && !labeled) {
+          if ((gt0 >> j) & 1) ulo = max(ulo, S.M[j]);
+          else if ((lt0 >> j) & 1) uhi = min(uhi, S.M[j]);
         }
-        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        S.hash_epoch = S.epoch;
-        S.hash_list = U;
-        S.hash_lo = us;
-        S.hash_hi = ue;
+      }
+      const int nU = S.uln[0];
+      const int us = ulo >= 0 ? lower_bound_wave(U, nU, ulo + 1) : 0;
+      const int ue = uhi != INT_MAX ? lower_bound_wave(U, nU, uhi) : nU;
+      kind = 4;
+      if (P.leaf_hash & 2) {
+        int words = 0;
+        bbase = 0;
+        if (ue > us) {
+          bbase = U[us] & ~31;
+          words = ((U[ue - 1] - bbase) >> 5) + 1;
+        }
+        // dense enough that a word holds an element per 128 ids on average (a
+        // sparse U is cheaper hashed, or probed element by element)
+        if (words <= kHash && words <= 4 * (ue - us) + 1) {
+          use_bits = true;
+          kind = 2;
+          bend = bbase + 32 * words;
+          lds_uint *bits = lds_u(S.htab);
+          for (int i = lane; i < words; i += kWave) bits[i] = 0u;
+          __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+          const bool lab_test = labeled && seg < 0;  // U may hold other labels (no segments)
+          const int lu = pl->label[u];
+          for (int idx = us + lane; idx < ue; idx += kWave) {
+            const int y = U[idx];
+            if (!lab_test || (int)P.labels[y] == lu)
+              __hip_atomic_fetch_or(bits + ((y - bbase) >> 5), 1u << (y & 31), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WAVEFRONT);
+          }
+          __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+          // injectivity: no path vertex before the owner is a candidate
+          if (lane < u - 1) {
+            const int m = S.M[lane];
+            if (m >= bbase && m < bend)
+              __hip_atomic_fetch_and(bits + ((m - bbase) >> 5), ~(1u << (m & 31)), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WAVEFRONT);
+          }
+          __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        }
+      }
+      // a hash is worth it when most parents' own lists are short next to U
+      // (else U is the better probe: bitmap or binary search in each
+      // parent's list); judged on the path's first batch
+      const int nact = __popcll(__ballot(act));
+      const int nshort = __popcll(__ballot(act && vn0 <= 4 * (ue - us)));
+      if (!use_bits && (P.leaf_hash & 1) && ue - us <= kHashMax && 2 * nshort >= nact) {
+        use_hash = true;
+        kind = 1;
+        hash_n = max(0, ue - us);
+        hash_build(lds_i(S.htab), U + us, ue - us, kHashLog);
+      }
+      if (lane == 0) {
+        S.tab_epoch = S.epoch;
+        S.tab_kind = kind;
+        S.tab_list = U;
+        S.tab_lo = us;
+        S.tab_hi = ue;
+        S.tab_base = bbase;
+        S.tab_end = bend;
       }
     }
+  }
+  if (use_bits) {
+    // Each owner x counts the elements of U's window in [a, b): the bitmap's
+    // span cut by x's own bound (GT: above x, LT: below x).  Units: the
+    // bitmap words of [a, b) when x has a neighbour bitmap and they are no
+    // more than x's list elements (row word AND LDS word, popcount), else the
+    // elements of x's list in [a, b) (one LDS bit each).  The units of all
+    // owners are flattened as in the pair path below.
+    taken = 0;
+    if (STATS && labeled) bytes += (unsigned)minfull;
+    const int ju = u - 1;
+    const bool c_gt = !labeled && ((pl->gt_mask[u] >> ju) & 1);
+    const bool c_lt = !labeled && ((pl->lt_mask[u] >> ju) & 1);
+    int a = bbase, b = bend;
+    if (c_gt) a = max(a, xa + 1);
+    if (c_lt) b = min(b, xa);
+    if (!act || b < a) b = a;
+    const int nw = b > a ? ((b - 1) >> 5) - (a >> 5) + 1 : 0;
+    const bool wm = act && vh0 >= 0 && nw <= vn0;
+    int st = 0, cnt = 0;
+    if (wm) {
+      st = a >> 5;
+      cnt = nw;
+    } else if (act && b > a) {
+      if (vn0 > kWave) {
+        st = lower_bound_i(vp0, vn0, a);
+        cnt = lower_bound_i(vp0, vn0, b) - st;
+      } else {
+        cnt = vn0;  // short list: bounds tested per element
+      }
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    int tk = __popcll(__ballot(act && incl <= P.leaf_pair_cap));
+    if (tk == 0) tk = 1;
+    taken = tk;
+    const bool mine = lane < tk;
+    if (!mine) cnt = 0;
+    const int T = __shfl(incl, tk - 1);
+    if (STATS) {
+      balg += wave_sum(mine ? bytes : 0ull);
+      next += (unsigned)tk;
+      count_op<STATS>(S, 2, (unsigned)T);
+      count_op<STATS>(S, 3, clock64() - t_setup);
+    }
+    const lds_uint *bits = lds_u(S.htab);
+    const int bw = bbase >> 5;
+    unsigned long long found = 0;
+    for (int q0 = 0; q0 < T; q0 += kWave) {
+      const int q = q0 + lane;
+      int o = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const int w = __shfl(incl, o + step - 1);
+        if (w <= q) o += step;
+      }
+      if (q >= T) o = 0;
+      const int i = q - (__shfl(incl, o) - __shfl(cnt, o));
+      const int ao = __shfl(a, o), bo = __shfl(b, o), so = __shfl(st, o);
+      const bool wmo = __shfl((int)wm, o) != 0;
+      const int ho = __shfl(vh0, o);
+      const int *lpo = reinterpret_cast<const int *>(__shfl(reinterpret_cast<unsigned long long>(vp0), o));
+      if (q < T) {
+        if (wmo) {
+          const int w = so + i;
+          unsigned v = P.hub_bits[(long long)ho * P.hub_words + w] & bits[w - bw];
+          const int s = ao - 32 * w, e = bo - 32 * w;
+          if (s > 0) v &= ~0u << s;
+          if (e < 32) v &= (1u << e) - 1u;
+          found += (unsigned)__popc(v);
+        } else {
+          const int y = lpo[so + i];
+          if (y >= ao && y < bo && ((bits[(y >> 5) - bw] >> (y & 31)) & 1u)) found++;
+        }
+      }
+    }
+    return wave_sum(found);
   }
   if (!use_hash && L > 0 && nu == 1 && hasA && !hasB && S.uln[0] <= 2 * kWave) {
     // One shared input U (C(dep), H(u) or N(M(b)), b <= L) of at most 128
@@ -940,6 +1205,10 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     const int y2 = two && kWave + lane < nU ? U[kWave + lane] : 0;
     const bool yv2 = two && kWave + lane < nU && shared_ok(y2);
     const int owners = i1 - i0;
+    if (STATS) {
+      count_op<STATS>(S, 2, (unsigned long long)owners * (unsigned)nU);
+      count_op<STATS>(S, 3, clock64() - t_setup);
+    }
     unsigned long long found = 0;
     for (int o0 = 0; o0 < owners; o0 += per) {
       const int o = o0 + slot;
@@ -1057,6 +1326,8 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
   if (STATS) {
     balg += wave_sum(mine ? bytes : 0ull);
     next += (unsigned)tk;
+    count_op<STATS>(S, 2, (unsigned)T);
+    count_op<STATS>(S, 3, clock64() - t_setup);
   }
   const int lab = pl->label[u];
   unsigned long long found = 0;
@@ -1089,7 +1360,7 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
     for (int i = 0; i < nu; i++) {
       if (i != po && ok) {
         const unsigned *h = S.uhb[i];
-        ok = use_hash ? hash_has(S.htab, y) : (h ? bit_test(h, y) : contains(S.ulp[i], S.uln[i], y));
+        ok = use_hash ? hash_has(lds_i(S.htab), y) : (h ? bit_test(h, y) : contains(S.ulp[i], S.uln[i], y));
       }
     }
     if (hasA) {
@@ -1124,6 +1395,7 @@ template <bool STATS, bool MERGE>
 __device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S,
                                                         int L, int i0, int i1, const int *lbase, int *hbase,
                                                         unsigned long long &balg, unsigned long long &next) {
+  count_op<STATS>(S, 7, 1);
   const int lane = lane_id();
   const int u = pl->k - 1;
   const int ju = u - 1;  // the owner's vertex
@@ -1164,76 +1436,137 @@ __device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const 
   const bool c_lt = !labeled && ((lt >> ju) & 1);
   const bool c_ne = !c_gt && !c_lt && ((same >> ju) & 1);
   // I as a sorted list base[0, len) less `nex` excluded path vertices ex[] that occur in it
-  const int *base;
-  int len;
+  const int *base = nullptr;
+  int len = 0;
   int ex[6];    // path vertices excluded from I (-1: none), at fixed positions
   int nex = 0;
-  int nin = 0;  // inputs of the leaf
-  int single_len = 0;
-  const int *single = nullptr;
-  if (hoisted) {
-    single = hbase + (long long)(u - 3) * P.dstride;
-    single_len = S.hl[u];
-    nin = 1;
-  } else if (dep >= 0) {
-    single = lbase + (long long)(dep - 2) * P.dstride;
-    single_len = S.len[dep - 1];
-    nin = 1 + pl->num_bn_sh[u];
-  } else {
-    nin = pl->num_bn[u];
-    if (nin == 1) single = adj_list(P, S.M[pl->bn[u][0]], seg, single_len);
-  }
-  const bool label_known = !labeled || seg >= 0 || dep >= 0 || hoisted;
+#pragma unroll
+  for (int j = 0; j < 6; j++) ex[j] = -1;
   int lo = -1, hi = INT_MAX;
 #pragma unroll
   for (int j = 0; j < 6; j++) {
-    ex[j] = -1;
     if (j < ju) {
       const int m = S.M[j];
       if (!labeled && ((gt >> j) & 1)) lo = max(lo, m);
       else if (!labeled && ((lt >> j) & 1)) hi = min(hi, m);
     }
   }
-  if (nin == 1 && label_known) {
-    int start = 0, stop = single_len;
-    if (lo >= 0) start = lower_bound_i(single, single_len, lo + 1);
-    if (hi != INT_MAX) stop = lower_bound_i(single, single_len, hi);
-    if (stop < start) stop = start;
-    base = single + start;
-    len = stop - start;
+  // I less its excluded vertices, staged ascending in this wave's LDS table
+  // when it fits (searched there, and kept for the later calls of the same
+  // path: I depends on M(0..k-3) only)
+  const bool staged = S.tab_kind == 3 && S.tab_epoch == S.epoch;
+  if (!staged) {
+    int nin = 0;  // inputs of the leaf
+    int single_len = 0;
+    const int *single = nullptr;
+    if (hoisted) {
+      single = hbase + (long long)(u - 3) * P.dstride;
+      single_len = S.hl[u];
+      nin = 1;
+    } else if (dep >= 0) {
+      single = lbase + (long long)(dep - 2) * P.dstride;
+      single_len = S.len[dep - 1];
+      nin = 1 + pl->num_bn_sh[u];
+    } else {
+      nin = pl->num_bn[u];
+      if (nin == 1) single = adj_list(P, S.M[pl->bn[u][0]], seg, single_len);
+    }
+    const bool label_known = !labeled || seg >= 0 || dep >= 0 || hoisted;
+    if (nin == 1 && label_known) {
+      int start = 0, stop = single_len;
+      if (lo >= 0) start = lower_bound_wave(single, single_len, lo + 1);
+      if (hi != INT_MAX) stop = lower_bound_wave(single, single_len, hi);
+      if (stop < start) stop = start;
+      base = single + start;
+      len = stop - start;
 #pragma unroll
-    for (int j = 0; j < 6; j++) {
-      if (j < ju && ((same >> j) & 1) && !(!labeled && (((gt | lt) >> j) & 1))) {
-        const int v = S.M[j];
-        if (v > lo && v < hi && contains(base, len, v)) {
-          ex[j] = v;
-          nex++;
+      for (int j = 0; j < 6; j++) {
+        if (j < ju && ((same >> j) & 1) && !(!labeled && (((gt | lt) >> j) & 1))) {
+          const int v = S.M[j];
+          if (v > lo && v < hi) {
+            const int pos = lower_bound_wave(base, len, v);
+            if (pos < len && base[pos] == v) {
+              ex[j] = v;
+              nex++;
+            }
+          }
         }
+      }
+    } else {
+      int *buf = hbase + (long long)(u - 3) * P.dstride;  // H(k-1) is not built: its slot is free
+      unsigned long long db = 0, dn = 0;
+      len = (int)extend<false, MERGE>(P, pl, S, u, buf, lbase, hbase, kMemo, false, db, dn);
+      base = buf;
+    }
+    if (len - nex <= kHash) {
+      lds_int *tab = lds_i(S.htab);
+      int n = 0;
+      for (int b0 = 0; b0 < len; b0 += kWave) {
+        const int i = b0 + lane;
+        const int y = i < len ? base[i] : 0;
+        bool keep = i < len;
+#pragma unroll
+        for (int j = 0; j < 6; j++) keep = keep && y != ex[j];
+        const unsigned long long m = __ballot(keep);
+        if (keep) tab[n + (int)mbcnt64(m)] = y;
+        n += __popcll(m);
+      }
+      __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      if (lane == 0) {
+        S.tab_kind = 3;
+        S.tab_epoch = S.epoch;
+        S.tab_lo = n;
+      }
+    }
+  }
+  // no condition ties the leaf to its owner: every owner counts all of I
+  if (!c_gt && !c_lt && !c_ne) return (unsigned long long)(staged ? S.tab_lo : len - nex) * (unsigned)owners;
+  unsigned long long found = 0;
+  if (staged || len - nex <= kHash) {
+    const lds_int *tab = lds_i(S.htab);
+    const int nI = staged ? S.tab_lo : len - nex;
+    for (int o0 = i0; o0 < i1; o0 += kWave) {
+      const int idx = o0 + lane;
+      if (idx < i1) {
+        const int x = lbase[(long long)(L - 1) * P.dstride + idx];
+        int c = nI;
+        if (c_gt || c_lt || (c_ne && x > lo && x < hi)) {
+          int p0 = 0, m = nI;  // first element >= x (GT: > x)
+          const int key = c_gt ? x + 1 : x;
+          while (m > 0) {
+            const int h = m >> 1;
+            if (tab[p0 + h] < key) {
+              p0 += h + 1;
+              m -= h + 1;
+            } else {
+              m = h;
+            }
+          }
+          if (c_gt) c = nI - p0;
+          else if (c_lt) c = p0;
+          else if (p0 < nI && tab[p0] == x) c--;  // x itself is no candidate (path vertices are never owners)
+        }
+        found += (unsigned)c;
       }
     }
   } else {
-    int *buf = hbase + (long long)(u - 3) * P.dstride;  // H(k-1) is not built: its slot is free
-    unsigned long long db = 0, dn = 0;
-    len = (int)extend<false, MERGE>(P, pl, S, u, buf, lbase, hbase, kMemo, false, db, dn);
-    base = buf;
-  }
-  int nI = len - nex;
-  unsigned long long found = 0;
-  for (int o0 = i0; o0 < i1; o0 += kWave) {
-    const int idx = o0 + lane;
-    if (idx < i1) {
-      const int x = lbase[(long long)(L - 1) * P.dstride + idx];
-      int c = nI;
-      if (c_gt || c_lt) {
-        const int pos = lower_bound_i(base, len, c_gt ? x + 1 : x);  // elements below the bound
-        c = c_gt ? len - pos : pos;
+    const int nI = len - nex;
+    for (int o0 = i0; o0 < i1; o0 += kWave) {
+      const int idx = o0 + lane;
+      if (idx < i1) {
+        const int x = lbase[(long long)(L - 1) * P.dstride + idx];
+        int c = nI;
+        if (c_gt || c_lt) {
+          const int pos = lower_bound_i(base, len, c_gt ? x + 1 : x);  // elements below the bound
+          c = c_gt ? len - pos : pos;
 #pragma unroll
-        for (int j = 0; j < 6; j++)
-          if (ex[j] >= 0 && (c_gt ? ex[j] > x : ex[j] < x)) c--;
-      } else if (c_ne && x > lo && x < hi) {
-        if (contains(base, len, x)) c--;  // x itself is no candidate (path vertices are never owners)
+          for (int j = 0; j < 6; j++)
+            if (ex[j] >= 0 && (c_gt ? ex[j] > x : ex[j] < x)) c--;
+        } else if (c_ne && x > lo && x < hi) {
+          if (contains(base, len, x)) c--;  // x itself is no candidate (path vertices are never owners)
+        }
+        found += (unsigned)c;
       }
-      found += (unsigned)c;
     }
   }
   return wave_sum(found);
@@ -1476,7 +1809,28 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   long long t0 = wall_clock64();
   long long tprog = t0;
   const bool may_donate = P.timeout_ticks >= 0;
+  // Free levels (timed kernel, counting): when vertex level+1 is free, the
+  // subtree count of the level's first iteration is measured and stands for
+  // each of its other iterations.  Nothing inside a measured subtree is
+  // donated (its count must all land here); the level's own untaken
+  // iterations may be, and are then measured by their receiver.
+  const int fmask = (!STATS && !STEAL && !P.emit_n) ? pl->free_mask : 0;
+  int mmask = 0;  // levels whose first iteration's subtree is being measured
+  // Nothing is donated, and no progress reported, before this call has
+  // claimed work of its own: a received task could otherwise be passed on
+  // whole before anyone works on it, from wave to wave without end (each
+  // hop would also keep the idle deadline from seeing a stall).
+  bool worked = false;
   for (;;) {
+    if ((mmask >> level) & 1) {
+      // back at a measuring level: the first iteration's subtree is complete
+      const unsigned long long sub = cnt - S.mbase[level];
+      int nrest = 0;
+      (void)level_claim(S, level, kIeMax, nrest, shared);
+      cnt += sub * (unsigned long long)(unsigned)nrest;
+      mmask &= ~(1 << level);
+      continue;
+    }
     const unsigned long long w = ie_read(S, level);
     if (ie_it(w) >= ie_end(w)) {
       if (level == floor) return;
@@ -1489,11 +1843,11 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     }
     if (level < P.donate_levels) {
       const long long now = wall_clock64();
-      if (now - tprog > P.prog_ticks) {  // tell idle waves that this one still advances
+      if (worked && now - tprog > P.prog_ticks) {  // tell idle waves that this one still advances
         tprog = now;
         if (lane_id() == 0) fadd(&P.ctrl->progress, 1ull);
       }
-      if (may_donate && now - t0 > P.timeout_ticks) {
+      if (may_donate && worked && now - t0 > P.timeout_ticks) {
         t0 = now;
         // donate only while idle waves outnumber the tasks already queued
         int idle = 0;
@@ -1505,7 +1859,8 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
         idle = uni(idle);
         if (idle > 0) {
           const long long tp = STATS ? clock64() : 0;
-          const int n = donate<STEAL>(P, S, pid, floor, level, L0r, L0c, idle);
+          const int dl = mmask ? min(level, __builtin_ctz(mmask)) : level;
+          const int n = donate<STEAL>(P, S, pid, floor, dl, L0r, L0c, idle);
           if (STATS) prof[5] += clock64() - tp;
           don += (unsigned)n;
           if (n > 0) continue;  // re-read this level: its end has moved
@@ -1517,6 +1872,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       int n = 0;
       const int it = level_claim(S, level, 1024, n, shared);
       if (n == 0) continue;
+      worked = true;
       const long long tp = STATS ? clock64() : 0;
       cnt += leaf_memo<STATS, MERGE>(P, pl, S, level, it, it + n, lbase, hbase, balg, next);
       if (STATS) prof[2] += clock64() - tp;
@@ -1526,9 +1882,12 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       int n = 0;
       const int it = level_claim(S, level, kWave, n, shared);
       if (n == 0) continue;  // a sibling took the rest
+      worked = true;
       int taken = 0;
       const long long tp = STATS ? clock64() : 0;
       cnt += leaf_batch<STATS>(P, pl, S, level, it, it + n, L0r, L0c, lbase, hbase, balg, next, taken);
+      count_op<STATS>(S, 0, 1);
+      count_op<STATS>(S, 1, (unsigned)taken);
       if (STATS) prof[2] += clock64() - tp;
       if (taken < n) level_unclaim(S, level, it + taken, it + n, shared);
       continue;
@@ -1536,6 +1895,11 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     int n1 = 0;
     const int it = level_claim(S, level, 1, n1, shared);
     if (n1 == 0) continue;
+    worked = true;
+    if ((fmask >> (level + 1)) & 1) {  // this iteration's subtree count stands for the level's others
+      if (lane_id() == 0) S.mbase[level] = cnt;
+      mmask |= 1 << level;
+    }
     if (level == 0) {
       S.M[0] = L0r[it];
       S.M[1] = L0c[it];
@@ -1685,8 +2049,9 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   S.tile = MERGE ? tiles + wib * 2 * (kTile + 4) : nullptr;
   S.htab = htabs + wib * kHash;
   S.epoch = 0;
-  S.hash_epoch = ~0u;
-  S.hash_list = nullptr;
+  S.tab_epoch = ~0u;
+  S.tab_kind = 0;
+  S.tab_list = nullptr;
   clear_hoists(S);
   if (lane < 8) S.ie[lane] = 0;
   S.readers = 0;
@@ -1698,6 +2063,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   int stat_pid = -1;
   bool is_idle = false;
   long long prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < 8) S.ctr[lane] = 0;
   const long long t_start = STATS ? clock64() : 0;
   const long long t_wave = wall_clock64();
   int cur_plan = (int)(gw % P.nplans);
@@ -1926,6 +2292,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
 #pragma unroll
     for (int i = 0; i < 7; i++) atomicAdd(&P.prof[i], (unsigned long long)prof[i]);
     atomicAdd(&P.prof[7], 1ull);
+    for (int i = 0; i < 8; i++) atomicAdd(&P.prof[8 + i], S.ctr[i]);
   }
 }
 
@@ -1986,7 +2353,7 @@ __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigne
   for (unsigned long long i = tid; i < (unsigned long long)nplans; i += nth) counts[i] = 0;
   for (unsigned long long i = tid; i < (unsigned long long)nplans * 4; i += nth) stats[i] = 0;
   for (unsigned long long i = tid; i < cap; i += nth) seq[i] = i;
-  for (unsigned long long i = tid; i < 8; i += nth) prof[i] = 0;
+  for (unsigned long long i = tid; i < (unsigned long long)kProf; i += nth) prof[i] = 0;
   for (unsigned long long i = tid; i < (unsigned long long)nplans * 16; i += nth) cursors[i] = 0;
 }
 
@@ -2076,7 +2443,7 @@ struct tdfs_ctx {
   tdfs_tuning tune;
   int stats;
   Launch main;              // used by tdfs_count / tdfs_count_batch
-  unsigned long long last_prof[8];
+  unsigned long long last_prof[kProf];
 };
 
 #define HIP_TRY(expr)                                                                        \
@@ -2107,6 +2474,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->steal = 2;
   t->leaf_memo = 1;
   t->leaf_hash = 1;
+  t->ext_hash = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -2135,7 +2503,8 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->idle_deadline_ms == 0) t->idle_deadline_ms = d.idle_deadline_ms;
   if (t->steal != 1) t->steal = 2;
   if (t->leaf_memo != 2) t->leaf_memo = 1;
-  if (t->leaf_hash != 2) t->leaf_hash = 1;
+  if (t->leaf_hash != 2 && t->leaf_hash != 3) t->leaf_hash = 1;
+  if (t->ext_hash != 2) t->ext_hash = 1;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -2144,9 +2513,9 @@ static int launch_init(Launch &L, int device) {
   HIP_TRY(hipEventCreate(&L.ev0));
   HIP_TRY(hipEventCreate(&L.ev1));
   HIP_TRY(hipMalloc(&L.ctrl, sizeof(Ctrl)));
-  HIP_TRY(hipMalloc(&L.prof, 8 * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&L.prof, kProf * sizeof(unsigned long long)));
   HIP_TRY(hipHostMalloc(&L.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc(&L.h_prof, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(&L.h_prof, kProf * sizeof(unsigned long long), hipHostMallocDefault));
   return TDFS_OK;
 }
 
@@ -2562,6 +2931,16 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
     for (int i = 0; i < n_in; i++) free_ = free_ && in[i] != u - 1;
     d.leaf_free[ri] = free_ ? 1 : 0;
   }
+  d.free_mask = 0;
+  d.pad2_ = 0;
+  for (int w = 2; p.labeled && w <= p.k - 3; w++) {
+    bool free_ = true;
+    for (int v = w + 1; v < p.k; v++) {
+      if (d.label[v] == d.label[w]) free_ = false;
+      for (int i = 0; i < p.num_bn[v]; i++) free_ = free_ && p.bn[v][i] != w;
+    }
+    if (free_) d.free_mask |= 1 << w;
+  }
   // H(u): the inputs of C(u) other than N(M(u-1)); worth a list of its own
   // when at least two of them remain to intersect
   for (int ri = 0; ri < 2; ri++) {
@@ -2771,7 +3150,8 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.reuse = t.reuse == 1 ? 1 : 0;
   P.leaf_batch = t.leaf_batch == 1 ? 1 : 0;
   P.leaf_memo = t.leaf_memo == 1 && P.leaf_batch ? 1 : 0;
-  P.leaf_hash = t.leaf_hash == 1 ? 1 : 0;
+  P.leaf_hash = t.leaf_hash == 1 ? 3 : (t.leaf_hash == 3 ? 1 : 0);
+  P.ext_hash = t.ext_hash == 1 ? 1 : 0;
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
   P.leaf_pair_cap = t.leaf_pair_cap;
@@ -2826,7 +3206,7 @@ static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
   HIP_TRY(hipMemcpyAsync(L.h_stats, L.stats, sizeof(unsigned long long) * 4 * P.nplans, hipMemcpyDeviceToHost,
                          L.stream));
   HIP_TRY(hipMemcpyAsync(L.h_ctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.stream));
-  HIP_TRY(hipMemcpyAsync(L.h_prof, L.prof, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.stream));
+  HIP_TRY(hipMemcpyAsync(L.h_prof, L.prof, kProf * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.stream));
   return TDFS_OK;
 }
 
@@ -2838,7 +3218,7 @@ static int launch_collect(const Launch &L, const tdfs_plan *plans, int nplans, d
     return fail(TDFS_E_HIP, "device scheduler error %d (%s)", e,
                 e == 4 ? "idle deadline: no wave reported progress" : "task ring handshake");
   }
-  if (prof_out) std::memcpy(prof_out, L.h_prof, 8 * sizeof(unsigned long long));
+  if (prof_out) std::memcpy(prof_out, L.h_prof, kProf * sizeof(unsigned long long));
   for (int i = 0; i < nplans; i++) {
     tdfs_result &r = out[i];
     std::memset(&r, 0, sizeof(r));
@@ -3313,6 +3693,12 @@ extern "C" int tdfs_ctx_last_profile(const tdfs_ctx *ctx, uint64_t *out8) {
   return TDFS_OK;
 }
 
+extern "C" int tdfs_ctx_last_counters(const tdfs_ctx *ctx, uint64_t *out8) {
+  if (!ctx || !out8) return fail(TDFS_E_ARG, "null argument");
+  for (int i = 0; i < 8; i++) out8[i] = ctx->last_prof[8 + i];
+  return TDFS_OK;
+}
+
 extern "C" int tdfs_count(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_plan *plan, int64_t edge_begin,
                           int64_t edge_end, tdfs_result *out) {
   return run_plans(ctx, g, plan, 1, edge_begin, edge_end, out);
@@ -3443,7 +3829,7 @@ extern "C" int tdfs_count_noalloc(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs
   L.ctrl = reinterpret_cast<Ctrl *>(take(sizeof(Ctrl)));
   L.counts = reinterpret_cast<unsigned long long *>(take(sizeof(unsigned long long)));
   L.stats = reinterpret_cast<unsigned long long *>(take(4 * sizeof(unsigned long long)));
-  L.prof = reinterpret_cast<unsigned long long *>(take(8 * sizeof(unsigned long long)));
+  L.prof = reinterpret_cast<unsigned long long *>(take(kProf * sizeof(unsigned long long)));
   L.cursors = reinterpret_cast<unsigned long long *>(take(16 * sizeof(unsigned long long)));
   L.plans = reinterpret_cast<DevPlan *>(take(sizeof(DevPlan)));
   L.plan_cap = 1;
@@ -3456,7 +3842,7 @@ extern "C" int tdfs_count_noalloc(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs
   L.wbuf_external = true;
   // host sides of the plan upload and result copies: this call's own frame
   DevPlan h_plan;
-  unsigned long long h_counts[1], h_stats[4], h_prof[8];
+  unsigned long long h_counts[1], h_stats[4], h_prof[kProf];
   Ctrl h_ctrl;
   L.h_plans = &h_plan;
   L.h_counts = h_counts;

@@ -75,6 +75,10 @@ def check(dev, dg, G, text, labeled, stats=True, **tune):
     if stats:
         assert r.alg_bytes == want_dev["alg_bytes"], (text, labeled, tune)
         assert r.extensions == want_dev["extensions"], (text, labeled, tune)
+        # the timed kernel (the one measured) on the same case
+        dev.set_stats(False)
+        r2 = dev.count(dg, p)
+        assert r2.raw == want["raw"], (text, labeled, tune, "timed kernel")
     return r
 
 
@@ -146,6 +150,49 @@ def test_sparse_random_patterns(dev):
     dg.free()
 
 
+def free_levels(plan):
+    """Vertices w (2 <= w <= k-3) that no later vertex has as a backward
+    neighbour or shares a label with (labeled plans)."""
+    k = plan.k
+    return [w for w in range(2, k - 2)
+            if plan.labeled and all(plan.label[v] != plan.label[w]
+                                    and w not in [plan.bn[v][i] for i in range(plan.num_bn[v])]
+                                    for v in range(w + 1, k))]
+
+
+def test_free_levels_timed_kernel(dev):
+    """Labeled patterns with free middle vertices: the timed kernel counts
+    one subtree per free level and multiplies; counts equal the oracle's,
+    also with donation forced at every check (nothing inside a measured
+    subtree may be donated, and a received task is never passed on before
+    it is worked on), with one-edge chunks and with the 5-wave kernel."""
+    G = T.CSRGraph.chung_lu(6000, 30000, 2.1, 600, 91).with_labels(4, 92)
+    dg = dev.upload(G)
+    base = dev.tuning()
+    rng = np.random.default_rng(93)
+    cases = []
+    for j in range(60):
+        k = int(rng.integers(5, 8))
+        m = int(rng.integers(k - 1, k + 3))
+        text = T.gen_pattern(k, m, 4, int(rng.integers(1 << 62)))
+        p = T.plan_pattern(text, 1)
+        if m >= k and free_levels(p):  # no trees: their counts keep the oracle busy for minutes
+            cases.append((text, p))
+        if len(cases) == 8:
+            break
+    assert len(cases) == 8
+    dev.set_stats(False)
+    for j, (text, p) in enumerate(cases):
+        want = O.count(G.rowptr, G.colidx, G.labels, O.plan(text, 1), threads=4)["raw"]
+        for mode in (dict(), dict(timeout_us=1, donate_levels=5), dict(chunk_edges=1, timeout_us=1),
+                     dict(waves_per_simd=5, timeout_us=1, donate_levels=5)):
+            say("free levels", j, free_levels(p), mode)
+            dev.set_tuning(**mode)
+            assert dev.count(dg, p).raw == want, (j, mode)
+            dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+    dg.free()
+
+
 def test_scheduling_modes_are_exact(dev):
     """Forced scheduler paths: donate at every check, a tiny task ring (full
     ring path), odd chunk sizes, reuse off; counts and algorithmic bytes are
@@ -192,6 +239,10 @@ def test_scheduling_modes_are_exact(dev):
         dict(leaf_memo=2, reuse=2),
         dict(leaf_hash=2),
         dict(leaf_hash=1, leaf_pair_cap=64, timeout_us=1, steal=1),
+        dict(leaf_hash=3),
+        dict(leaf_hash=3, leaf_pair_cap=64, timeout_us=1),
+        dict(ext_hash=2),
+        dict(ext_hash=2, leaf_hash=2, leaf_memo=2),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
         for labeled in (0, 1):
