@@ -40,7 +40,7 @@ def noalloc(dev, dg, plan, lo=0, hi=-1):
 
 def test_noalloc_equals_count():
     dev = T.Device(0)
-    big = T.CSRGraph.chung_lu(60000, 400000, 2.1, 3000, 17).with_labels(3, 17)
+    big = T.CSRGraph.chung_lu(30000, 150000, 2.1, 1500, 17).with_labels(3, 17)
     small = T.CSRGraph.chung_lu(3000, 12000, 2.1, 200, 18).with_labels(3, 18)
     for gname, G in (("big", big), ("small", small)):  # the second call's input is smaller
         dg = dev.upload(G)
