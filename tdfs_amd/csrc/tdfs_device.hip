@@ -97,7 +97,9 @@ struct DevPlan {
   // backward neighbour or shares its label (labeled plans), so no later
   // candidate set depends on M(w) and every M(w) has the same subtree count
   int free_mask;
-  int pad2_;
+  // bit j (j <= k-2): M(j) lies in the leaf's base set (the intersection of
+  // N(M(b)), b in BN(k-1)) in every embedding -- j is adjacent to every such b
+  int leaf_in;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -1454,6 +1456,15 @@ __device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const 
   // I less its excluded vertices, staged ascending in this wave's LDS table
   // when it fits (searched there, and kept for the later calls of the same
   // path: I depends on M(0..k-3) only)
+  // The leaf's input is one raw adjacency list N(M(b0)) (no reuse, no
+  // hoisting): the plan then tells which path vertices lie in it in every
+  // embedding (adjacent to b0) -- no search for them -- and an owner test
+  // "x in I" is a window compare (owner adjacent to b0) or, with b0's
+  // vertex a hub, one bitmap load.
+  const bool raw = !hoisted && dep < 0 && pl->num_bn[u] == 1 && (!labeled || seg >= 0);
+  const bool owner_in = raw && ((pl->leaf_in >> ju) & 1);
+  // owners need no search in I
+  const bool fast_ne = c_ne && (owner_in || (raw && P.hub_idx[uni(S.M[pl->bn[u][0]])] >= 0));
   const bool staged = S.tab_kind == 3 && S.tab_epoch == S.epoch;
   if (!staged) {
     int nin = 0;  // inputs of the leaf
@@ -1484,8 +1495,16 @@ __device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const 
         if (j < ju && ((same >> j) & 1) && !(!labeled && (((gt | lt) >> j) & 1))) {
           const int v = S.M[j];
           if (v > lo && v < hi) {
-            const int pos = lower_bound_wave(base, len, v);
-            if (pos < len && base[pos] == v) {
+            bool in;
+            if (raw && ((pl->leaf_in >> j) & 1)) {
+              in = true;  // adjacent to b0 in the pattern
+            } else if (raw && pl->bn[u][0] == j) {
+              in = false;  // b0 itself: no self-loops
+            } else {
+              const int pos = lower_bound_wave(base, len, v);
+              in = pos < len && base[pos] == v;
+            }
+            if (in) {
               ex[j] = v;
               nex++;
             }
@@ -1498,7 +1517,7 @@ __device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const 
       len = (int)extend<false, MERGE>(P, pl, S, u, buf, lbase, hbase, kMemo, false, db, dn);
       base = buf;
     }
-    if (len - nex <= kHash) {
+    if (len - nex <= kHash && !fast_ne) {
       lds_int *tab = lds_i(S.htab);
       int n = 0;
       for (int b0 = 0; b0 < len; b0 += kWave) {
@@ -1522,6 +1541,21 @@ __device__ __forceinline__ unsigned long long leaf_memo(const KParams &P, const 
   // no condition ties the leaf to its owner: every owner counts all of I
   if (!c_gt && !c_lt && !c_ne) return (unsigned long long)(staged ? S.tab_lo : len - nex) * (unsigned)owners;
   unsigned long long found = 0;
+  if (fast_ne && !staged) {
+    // NE only: x is in I when it lies in the window and in N(tv)
+    const int nI = len - nex;
+    if (owner_in && lo < 0 && hi == INT_MAX) return (unsigned long long)(nI - 1) * (unsigned)owners;
+    const unsigned *trow = owner_in ? nullptr : hub_row(P, uni(S.M[pl->bn[u][0]]));
+    for (int o0 = i0; o0 < i1; o0 += kWave) {
+      const int idx = o0 + lane;
+      if (idx < i1) {
+        const int x = lbase[(long long)(L - 1) * P.dstride + idx];
+        const bool inI = x > lo && x < hi && (owner_in || bit_test(trow, x));
+        found += (unsigned)(nI - (inI ? 1 : 0));
+      }
+    }
+    return wave_sum(found);
+  }
   if (staged || len - nex <= kHash) {
     const lds_int *tab = lds_i(S.htab);
     const int nI = staged ? S.tab_lo : len - nex;
@@ -2296,10 +2330,10 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   }
 }
 
-// Timed / statistics kernel: registers as the compiler allocates them
-// (3 waves per SIMD, no spills).
+// Timed / statistics kernel: registers as the compiler allocates them, the
+// timed one bounded to 3 waves per SIMD (168 VGPRs).
 template <bool STATS>
-__global__ void __launch_bounds__(kBlock) tdfs_match_kernel(KParams P) {
+__global__ void __launch_bounds__(kBlock, STATS ? 1 : 3) tdfs_match_kernel(KParams P) {
   match_body<STATS, false>(P);
 }
 
@@ -2932,7 +2966,13 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
     d.leaf_free[ri] = free_ ? 1 : 0;
   }
   d.free_mask = 0;
-  d.pad2_ = 0;
+  d.leaf_in = 0;
+  for (int j = 0; j + 1 < p.k; j++) {
+    const int u = p.k - 1;
+    bool in = p.num_bn[u] > 0;
+    for (int i = 0; i < p.num_bn[u]; i++) in = in && p.adj[j][p.bn[u][i]] != 0;  // never for j in BN(u)
+    if (in) d.leaf_in |= 1 << j;
+  }
   for (int w = 2; p.labeled && w <= p.k - 3; w++) {
     bool free_ = true;
     for (int v = w + 1; v < p.k; v++) {
