@@ -759,9 +759,19 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
       }
     }
   }
+  // A probe of at most 64 elements is loaded once: its window comes from
+  // ballots over the lanes, and the probe loop takes its elements from them.
+  const bool small = pn <= kWave;
+  int xr = 0;
   int start = 0, stop = pn;
-  if (lo >= 0) start = lower_bound_wave(pp, pn, lo + 1);
-  if (hi != INT_MAX) stop = lower_bound_wave(pp, pn, hi);
+  if (small) {
+    xr = lane < pn ? pp[lane] : INT_MAX;
+    if (lo >= 0) start = __popcll(__ballot(lane < pn && xr <= lo));
+    if (hi != INT_MAX) stop = __popcll(__ballot(lane < pn && xr < hi));
+  } else {
+    if (lo >= 0) start = lower_bound_wave(pp, pn, lo + 1);
+    if (hi != INT_MAX) stop = lower_bound_wave(pp, pn, hi);
+  }
   unsigned n = 0;
   // merge path: two inputs, both long, neither more than 8x the other
   int mo = -1, ms = 0, me = 0;
@@ -793,8 +803,12 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         if ((nemask >> j) & 1) {
           const int v = nev[j];
           if (v > lo && v < hi) {
-            const int pos = lower_bound_wave(pp + start, stop - start, v);
-            if (pos < stop - start && pp[start + pos] == v) n--;
+            if (small) {
+              if (__ballot(lane >= start && lane < stop && xr == v)) n--;
+            } else {
+              const int pos = lower_bound_wave(pp + start, stop - start, v);
+              if (pos < stop - start && pp[start + pos] == v) n--;
+            }
           }
         }
       }
@@ -830,9 +844,10 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
       for (int base = start; base < stop; base += kWave) {
         const int idx = base + lane;
         bool ok = idx < stop;
+        const int xs = small ? __shfl(xr, min(idx, kWave - 1)) : 0;  // all lanes take part in the shuffle
         int x = 0;
         if (ok) {
-          x = pp[idx];
+          x = small ? xs : pp[idx];
 #pragma unroll
           for (int j = 0; j < 6; j++)
             if ((nemask >> j) & 1) ok = ok && (x != nev[j]);
@@ -1021,15 +1036,27 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
         }
       }
       const int nU = S.uln[0];
-      const int us = ulo >= 0 ? lower_bound_wave(U, nU, ulo + 1) : 0;
-      const int ue = uhi != INT_MAX ? lower_bound_wave(U, nU, uhi) : nU;
+      // U of at most 64 elements is loaded once (window by ballots, ends and
+      // bitmap from the lanes)
+      const bool smallU = nU <= kWave;
+      const int yU = smallU && lane < nU ? U[lane] : INT_MAX;
+      int us, ue;
+      if (smallU) {
+        us = ulo >= 0 ? __popcll(__ballot(lane < nU && yU <= ulo)) : 0;
+        ue = uhi != INT_MAX ? __popcll(__ballot(lane < nU && yU < uhi)) : nU;
+      } else {
+        us = ulo >= 0 ? lower_bound_wave(U, nU, ulo + 1) : 0;
+        ue = uhi != INT_MAX ? lower_bound_wave(U, nU, uhi) : nU;
+      }
       kind = 4;
       if (P.leaf_hash & 2) {
         int words = 0;
         bbase = 0;
         if (ue > us) {
-          bbase = U[us] & ~31;
-          words = ((U[ue - 1] - bbase) >> 5) + 1;
+          const int first = smallU ? __shfl(yU, us) : U[us];
+          const int last = smallU ? __shfl(yU, ue - 1) : U[ue - 1];
+          bbase = first & ~31;
+          words = ((last - bbase) >> 5) + 1;
         }
         // dense enough that a word holds an element per 128 ids on average (a
         // sparse U is cheaper hashed, or probed element by element)
@@ -1042,11 +1069,15 @@ __device__ __forceinline__ unsigned long long leaf_batch(const KParams &P, const
           __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
           const bool lab_test = labeled && seg < 0;  // U may hold other labels (no segments)
           const int lu = pl->label[u];
-          for (int idx = us + lane; idx < ue; idx += kWave) {
-            const int y = U[idx];
+          auto set_bit = [&](int y) {
             if (!lab_test || (int)P.labels[y] == lu)
               __hip_atomic_fetch_or(bits + ((y - bbase) >> 5), 1u << (y & 31), __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_WAVEFRONT);
+          };
+          if (smallU) {
+            if (lane >= us && lane < ue) set_bit(yU);  // lane i holds U[i]
+          } else {
+            for (int idx = us + lane; idx < ue; idx += kWave) set_bit(U[idx]);
           }
           __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
           // injectivity: no path vertex before the owner is a candidate
