@@ -40,6 +40,7 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {0: "tdfs_match_kernel", 4: "tdfs_match_kernel_w4", 5: "tdfs_match_kernel_w5",
                 6: "tdfs_match_kernel_w6", 8: "tdfs_match_kernel_w8"}
 SERVE_RATES = (50.0, 200.0, 500.0, 1000.0, 2000.0)
+SERVE_POINT = {"slots": 8, "blocks_per_cu": 1, "max_batch": 4, "waves_per_simd": 5}
 
 
 def load_workload(name: str):
@@ -258,8 +259,12 @@ def serve_leg(queries: int, rates, time_limit_ms: int) -> dict:
     dg = dev.upload(g)
     plans = [T.plan_pattern(q["text"], q["labeled"]) for q in qs]
     alone = [int(dev.count(dg, p).count) for p in plans]
-    dev.set_tuning(time_limit_ms=time_limit_ms)
-    sv = serve.Server(dev, dg, slots=4, blocks_per_cu=1, max_batch=8)
+    # serving point: 8 slots of one workgroup per CU, the 5-wave kernel, at
+    # most 4 queries per launch (swept over slots, occupancy and batch size
+    # with python -m tdfs_amd.serve on this stream: 37-39 queries/s)
+    dev.set_tuning(time_limit_ms=time_limit_ms, waves_per_simd=SERVE_POINT["waves_per_simd"])
+    sv = serve.Server(dev, dg, slots=SERVE_POINT["slots"], blocks_per_cu=SERVE_POINT["blocks_per_cu"],
+                      max_batch=SERVE_POINT["max_batch"])
     rows = []
     for rate in rates:
         r = serve.run_stream(sv, plans, rate, 50 + int(rate))
@@ -276,8 +281,7 @@ def serve_leg(queries: int, rates, time_limit_ms: int) -> dict:
     return {"workload": "c5: Chung-Lu n=4M m=34M gamma=2.3 seed 5, 8 labels; "
                         f"{len(qs)} committed queries (k {dict(sorted(ks.items()))}, "
                         f"{sum(q['labeled'] for q in qs)} labeled), Poisson arrivals",
-            "server": {"slots": 4, "blocks_per_cu": 1, "max_batch": 8, "admission": "DFS-stack memory arena",
-                       "time_limit_ms": time_limit_ms},
+            "server": {**SERVE_POINT, "admission": "DFS-stack memory arena", "time_limit_ms": time_limit_ms},
             "streams": rows}
 
 
