@@ -90,8 +90,11 @@ typedef struct tdfs_tuning {
                                 the result cannot depend on the numbering (default); 2: off.  Read at upload */
   int32_t time_limit_ms;     /* > 0: work still pending this long into a launch is dropped and the plans it
                                 belonged to are flagged TDFS_INCOMPLETE (default 0: no limit) */
-  int32_t cursor_order;      /* level-0 order of a batch: 1 one cursor over (edge chunk, plan), all plans
-                                sweep the graph together (default); 2 one cursor per plan (serving) */
+  int32_t cursor_order;      /* level-0 order of a batch: 1 (default) cursors over (edge chunk, plan): all
+                                plans sweep the same region of the graph together, with one cursor per XCD
+                                (chunk c goes to workgroups on XCD c mod 8) so that a chunk's plans share
+                                that XCD's L2; 3 the same with one cursor for the whole device; 2 one
+                                cursor per plan (serving) */
   int32_t waves_per_simd;    /* register budget of the timed kernel: 0 as compiled (default, no spills),
                                 4, 5, 6 or 8 waves per SIMD (spills) */
   int32_t hoist_min;         /* the inputs of C(u) that do not involve M(u-1) are intersected once per

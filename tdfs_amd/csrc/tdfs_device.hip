@@ -59,7 +59,8 @@ constexpr int kMaxSegLabels = 64;          // label segments are built for at mo
 constexpr int kTile = 512;                 // merge-path diagonals per round (8 per lane)
 constexpr int kHashLog = 10;               // leaf hash: 1,024 LDS slots per wave
 constexpr int kHash = 1 << kHashLog;
-constexpr int kProf = 16;                  // profile words per launch (8 cycle / wave figures, 8 counters)
+constexpr int kProf = 16;
+constexpr int kXcds = 8;                   // XCDs of an MI355X: interleaved level-0 cursors per launch                  // profile words per launch (8 cycle / wave figures, 8 counters)
 constexpr int kHashMax = kHash / 2;        // shared lists of at most this many elements are hashed
 
 // Per-pattern plan in the device's layout; read with wave-uniform loads.
@@ -159,7 +160,8 @@ struct KParams {
   long long edge_lo, edge_hi;
   unsigned long long span;      // edges per plan: edge_hi - edge_lo
   unsigned long long *cursors;  // per plan (128-byte stride): next level-0 offset
-  int interleave;               // 1: one cursor over (chunk, plan); 0: per-plan cursors
+  int interleave;               // 1: cursors over (chunk, plan); 0: per-plan cursors
+  int xcd_groups;               // interleaved: cursors, one per group of chunks (8: one per XCD)
   // per-wave workspace: level-0 pairs, then C(2..k-2), then H(3..k-1) at hoff
   int *wbuf;
   long long wstride;
@@ -2133,6 +2135,10 @@ __device__ __forceinline__ void match_body(const KParams &P) {
   const long long t_wave = wall_clock64();
   int cur_plan = (int)(gw % P.nplans);
   int plans_left = P.nplans;
+  // interleaved cursors: this workgroup's group (its XCD when workgroups are
+  // dealt to the XCDs in turn), and how many groups it has not found spent
+  int xg = (int)(blockIdx.x % (unsigned)P.xcd_groups);
+  int xleft = P.xcd_groups;
 
   for (;;) {
     // ---- acquire work: ring first, then the edge cursors, then wait ----
@@ -2146,18 +2152,25 @@ __device__ __forceinline__ void match_body(const KParams &P) {
       kind = 2;
     } else {
       unsigned long long pos = ~0ull;
-      if (P.interleave) {
-        // one cursor over (edge chunk, plan) pairs: all plans sweep the same
-        // region of the graph together (shared L2 / MALL residency)
-        if (lane == 0) pos = fadd(&P.cursors[0], (unsigned long long)P.chunk);
+      while (P.interleave && plans_left > 0) {
+        // cursor g hands out the edge chunks ci = g (mod groups), each to
+        // every plan in turn: all plans sweep the same region of the graph
+        // together, and with one group per XCD a chunk's plans run on one
+        // XCD and share its L2 (the lists and bitmaps of the chunk's
+        // vertices); a spent group sends its waves on to the next one
+        if (lane == 0) pos = fadd(&P.cursors[(long long)xg * 16], (unsigned long long)P.chunk);
         pos = uni64(pos);
         const unsigned long long c = pos / (unsigned long long)P.chunk;
-        const unsigned long long ci = c / (unsigned long long)P.nplans;
+        const unsigned long long ci = c / (unsigned long long)P.nplans * (unsigned)P.xcd_groups + (unsigned)xg;
         if (ci * (unsigned long long)P.chunk < P.span) {
           cur_plan = (int)(c % (unsigned long long)P.nplans);
           pos = ci * (unsigned long long)P.chunk;
-        } else {
+          break;
+        }
+        if (--xleft == 0) {
           plans_left = 0;
+        } else {
+          xg = xg + 1 == P.xcd_groups ? 0 : xg + 1;
         }
       }
       // per-plan cursors: this wave stays on its plan until that plan's
@@ -2410,7 +2423,7 @@ __global__ void tdfs_hub_build_kernel(const long long *rowptr, const int *colidx
 // that captured launch sequences hold no memset nodes).
 __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigned long long *stats, int nplans,
                                  unsigned long long *seq, unsigned long long cap, unsigned long long *prof,
-                                 unsigned long long *cursors) {
+                                 unsigned long long *cursors, int ncursors) {
   const unsigned long long tid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned long long nth = (unsigned long long)gridDim.x * blockDim.x;
   int *cw = reinterpret_cast<int *>(ctrl);
@@ -2419,7 +2432,7 @@ __global__ void tdfs_init_kernel(Ctrl *ctrl, unsigned long long *counts, unsigne
   for (unsigned long long i = tid; i < (unsigned long long)nplans * 4; i += nth) stats[i] = 0;
   for (unsigned long long i = tid; i < cap; i += nth) seq[i] = i;
   for (unsigned long long i = tid; i < (unsigned long long)kProf; i += nth) prof[i] = 0;
-  for (unsigned long long i = tid; i < (unsigned long long)nplans * 16; i += nth) cursors[i] = 0;
+  for (unsigned long long i = tid; i < (unsigned long long)ncursors * 16; i += nth) cursors[i] = 0;
 }
 
 }  // namespace
@@ -3089,7 +3102,7 @@ static int launch_reserve(Launch &L, int nplans, long long waves, long long wstr
     L.h_plans = nullptr;
     L.plan_cap = 0;
     HIP_TRY(hipMalloc(&L.plans, sizeof(DevPlan) * nplans));
-    HIP_TRY(hipMalloc(&L.cursors, sizeof(unsigned long long) * 16 * nplans));
+    HIP_TRY(hipMalloc(&L.cursors, sizeof(unsigned long long) * 16 * std::max(nplans, kXcds)));
     HIP_TRY(hipMalloc(&L.counts, sizeof(unsigned long long) * nplans));
     HIP_TRY(hipMalloc(&L.stats, sizeof(unsigned long long) * 4 * nplans));
     HIP_TRY(hipHostMalloc(&L.h_counts, sizeof(unsigned long long) * nplans, hipHostMallocDefault));
@@ -3200,7 +3213,8 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.edge_hi = edge_end;
   P.span = (unsigned long long)span;
   P.cursors = L.cursors;
-  P.interleave = (t.cursor_order == 1 && !ranges) ? 1 : 0;
+  P.interleave = ((t.cursor_order == 1 || t.cursor_order == 3) && !ranges) ? 1 : 0;
+  P.xcd_groups = P.interleave && t.cursor_order == 1 && nplans > 1 ? kXcds : 1;
   P.wbuf = L.wbuf;
   P.wstride = wstride;
   P.hoff = 2ll * kChunkMax + (long long)std::max(0, kmax - 3) * dstride;
@@ -3237,7 +3251,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
 static int launch_enqueue(Launch &L, const KParams &P, bool stats) {
   const int init_blocks = (int)std::min<unsigned long long>(1024, (L.q_cap + 255) / 256);
   hipLaunchKernelGGL(tdfs_init_kernel, dim3(init_blocks), dim3(256), 0, L.stream, L.ctrl, L.counts, L.stats,
-                     P.nplans, L.q_seq, L.q_cap, L.prof, L.cursors);
+                     P.nplans, L.q_seq, L.q_cap, L.prof, L.cursors, std::max(P.nplans, P.xcd_groups));
   HIP_TRY(hipGetLastError());
   if (P.merge_min) {
     if (stats)

@@ -230,6 +230,7 @@ def test_scheduling_modes_are_exact(dev):
         dict(leaf_pair_cap=64, timeout_us=1),
         dict(leaf_pair_cap=100000),
         dict(cursor_order=2, timeout_us=1, chunk_edges=16),
+        dict(cursor_order=3),
         dict(hoist_min=-1),
         dict(hoist_min=1, timeout_us=1, donate_levels=5),
         dict(hoist_min=1, reuse=2),
@@ -486,13 +487,20 @@ def test_batch_equals_alone(dev):
     dev.set_tuning(steal=2, merge_min=8)
     batch5 = dev.count_batch(dg, plans)
     dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
-    dev.set_tuning(cursor_order=1, merge_min=0)
+    say("batch with one cursor for the device, and timed")
+    dev.set_tuning(cursor_order=3, merge_min=0, steal=2)
+    batch6 = dev.count_batch(dg, plans)
+    dev.set_stats(False)
+    dev.set_tuning(cursor_order=1)
+    timed = dev.count_batch(dg, plans)
+    dev.set_stats(True)
     for j, p in enumerate(plans):
         say("alone", j)
         alone = dev.count(dg, p)
-        for b in (batch, batch2, batch3, batch4, batch5):
+        for b in (batch, batch2, batch3, batch4, batch5, batch6):
             assert alone.raw == b[j].raw, j
             assert alone.alg_bytes == b[j].alg_bytes, j
+        assert alone.raw == timed[j].raw, j
 
 
 def test_graph_replays_match_eager(dev):
