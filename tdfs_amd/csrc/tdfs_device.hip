@@ -101,6 +101,14 @@ struct DevPlan {
   // bit j (j <= k-2): M(j) lies in the leaf's base set (the intersection of
   // N(M(b)), b in BN(k-1)) in every embedding -- j is adjacent to every such b
   int leaf_in;
+  // tail: the last tail_t vertices (tail_t >= 2, 0: none) have one backward
+  // neighbour set, are pairwise non-adjacent and are either totally ordered
+  // by the symmetry conditions and otherwise bound only by injectivity
+  // (unlabeled: their completions are the tail_t-subsets of C(k - tail_t))
+  // or share one label (labeled: the ordered tail_t-tuples of distinct
+  // elements of C(k - tail_t))
+  int tail_t;
+  int pad3_;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -1986,6 +1994,24 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       return;
     }
     int *out = lbase + (long long)level * P.dstride;  // list of level+1
+    if (!STATS && !P.emit_n && pl->tail_t && u == k - pl->tail_t) {
+      // the tail's completions, counted from |C(u)| (timed kernel, counting)
+      const unsigned long long c = (unsigned long long)extend<false, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode,
+                                                                          false, balg, next);
+      const int t = pl->tail_t;
+      const unsigned long long lim = t == 2 ? 3000000000ull : (t == 3 ? 2000000ull : (t == 4 ? 50000ull : 6000ull));
+      if (c <= lim) {  // n^t fits 63 bits
+        unsigned long long f = 1;
+        for (int i = 0; i < t; i++) f *= c >= (unsigned)i ? c - i : 0ull;  // n (n-1) ... (n-t+1)
+        if (!pl->labeled) {
+          unsigned long long fac = 1;
+          for (int i = 2; i <= t; i++) fac *= (unsigned)i;
+          f /= fac;  // C(n, t)
+        }
+        cnt += f;
+        continue;
+      }
+    }
     const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
     if (n > 0) {
       S.len[level + 1] = n;
@@ -3011,6 +3037,35 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
   }
   d.free_mask = 0;
   d.leaf_in = 0;
+  d.tail_t = 0;
+  d.pad3_ = 0;
+  for (int t = 2; t <= p.k - 2; t++) {  // the tail starts at u0 = k - t >= 2
+    const int u0 = p.k - t;
+    bool ok = true;
+    for (int v = u0; v < p.k && ok; v++) {
+      ok = p.num_bn[v] == p.num_bn[u0];
+      for (int i = 0; ok && i < p.num_bn[v]; i++) ok = p.bn[v][i] == p.bn[u0][i];  // ascending lists
+      for (int w = u0; ok && w < p.k; w++) ok = w == v || p.adj[v][w] == 0;
+      if (ok && p.labeled) ok = p.label[v] == p.label[u0];
+      for (int j = 0; ok && !p.labeled && j < u0; j++) ok = p.cond[v][j] == TDFS_NE;
+    }
+    if (ok && !p.labeled) {
+      // every pair of tail vertices ordered (transitive closure of LT / GT)
+      bool less[kMaxK][kMaxK] = {};
+      for (int v = u0; v < p.k; v++)
+        for (int j = u0; j < v; j++) {
+          if (p.cond[v][j] == TDFS_LT) less[v][j] = true;
+          if (p.cond[v][j] == TDFS_GT) less[j][v] = true;
+        }
+      for (int m = u0; m < p.k; m++)
+        for (int a = u0; a < p.k; a++)
+          for (int b = u0; b < p.k; b++)
+            if (less[a][m] && less[m][b]) less[a][b] = true;
+      for (int a = u0; a < p.k && ok; a++)
+        for (int b = a + 1; b < p.k && ok; b++) ok = less[a][b] || less[b][a];
+    }
+    if (ok) d.tail_t = t;
+  }
   for (int j = 0; j + 1 < p.k; j++) {
     const int u = p.k - 1;
     bool in = p.num_bn[u] > 0;

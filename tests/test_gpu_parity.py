@@ -205,6 +205,32 @@ def test_free_levels_timed_kernel(dev):
     dg.free()
 
 
+def test_tails_timed_kernel(dev):
+    """Patterns whose last vertices share their backward neighbours, are
+    pairwise non-adjacent and are totally ordered (unlabeled) or share a
+    label (labeled): the timed kernel counts their completions as C(n, t) or
+    n!/(n-t)! of the first one's candidates; counts equal the oracle's,
+    with donation forced too."""
+    G = T.CSRGraph.chung_lu(5000, 25000, 2.1, 400, 95).with_labels(2, 96)
+    dg = dev.upload(G)
+    base = dev.tuning()
+    star4 = "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n"
+    book3 = "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 0 4 1\ne 1 2 1\ne 1 3 1\ne 1 4 1\n"
+    k23 = "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 2 1 1\ne 0 3 1\ne 2 3 1\ne 0 4 1\ne 2 4 1\n"
+    tri_tail = "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 0 2 1\ne 1 2 1\ne 2 3 1\ne 2 4 1\n"
+    dev.set_stats(False)
+    for name, text in (("star4", star4), ("book3", book3), ("k23", k23), ("triangle+2 pendants", tri_tail)):
+        for labeled in (0, 1):
+            p = T.plan_pattern(text, labeled)
+            want = O.count(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(text, labeled), threads=4)["raw"]
+            for mode in (dict(), dict(timeout_us=1, donate_levels=5)):
+                say("tail", name, labeled, mode)
+                dev.set_tuning(**mode)
+                assert dev.count(dg, p).raw == want, (name, labeled, mode)
+                dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+    dg.free()
+
+
 def test_scheduling_modes_are_exact(dev):
     """Forced scheduler paths: donate at every check, a tiny task ring (full
     ring path), odd chunk sizes, reuse off; counts and algorithmic bytes are
