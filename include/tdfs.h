@@ -126,6 +126,13 @@ typedef struct tdfs_tuning {
                                 neighbour bitmap and at most 512 elements in an LDS hash table, so each
                                 candidate's membership test is an LDS lookup instead of a binary search
                                 in HBM; 2: off */
+  int32_t tails;             /* 1 (default): the timed kernel counts closing vertices without enumerating
+                                them -- t >= 2 pairwise non-adjacent last vertices with one backward-
+                                neighbour set, totally ordered (unlabeled) or of one label, as C(n, t)
+                                or n!/(n-t)! of the first one's n candidates; a closing triangle over one
+                                set of earlier neighbours (cliques k >= 5) as the triangles among the
+                                first one's candidates when they are at most 64 vertices with neighbour
+                                bitmaps; 2: off */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

@@ -59,6 +59,7 @@ class Tuning(C.Structure):
         ("leaf_memo", C.c_int32),
         ("leaf_hash", C.c_int32),
         ("ext_hash", C.c_int32),
+        ("tails", C.c_int32),
     ]
 
 

@@ -108,7 +108,10 @@ struct DevPlan {
   // or share one label (labeled: the ordered tail_t-tuples of distinct
   // elements of C(k - tail_t))
   int tail_t;
-  int pad3_;
+  // 1: the last three vertices form a triangle over one set of earlier
+  // neighbours, totally ordered and otherwise bound only by injectivity
+  // (unlabeled): a path's completions are the triangles of G[C(k-3)]
+  int tri_tail;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -197,6 +200,7 @@ struct KParams {
   int leaf_hash;             // bit 0: a short shared leaf input may be hashed in LDS for membership tests;
                              // bit 1: a dense one may be held as an LDS bitmap
   int ext_hash;              // 1: an extension hashes one short non-probe input in LDS for membership tests
+  int tails;                 // 1: closing vertices counted without enumeration (timed kernel)
   long long limit_ticks;     // > 0: work still pending this long after a wave started is dropped
   int variant;               // host side only: 0 default kernel, 4 / 8 the kernel compiled for that occupancy
   int leaf_pair_cap;         // (owner, candidate) pairs per leaf batch
@@ -1870,6 +1874,31 @@ __device__ __forceinline__ void clear_hoists(WaveLds &S) {
   for (int u = 0; u < 8; u++) S.hv[u] = 0;
 }
 
+// Triangles of G[S] for a sorted list S of at most 64 vertices that all
+// have neighbour bitmaps: lane i holds S[i] and the mask R_i of the indices
+// j > i with S[j] adjacent to it (bits of its own row: the few lines of the
+// row that S's ids fall in), and each triangle i < j < z is counted once as a
+// bit of R_i & R_j, j in R_i.  Returns -1 (not counted) when some element has
+// no bitmap.
+__device__ __forceinline__ long long triangles_small(const KParams &P, const int *Sl, int s) {
+  const int lane = lane_id();
+  const int x = lane < s ? Sl[lane] : 0;
+  const int h = lane < s ? P.hub_idx[x] : 0;
+  if (__ballot(lane < s && h < 0)) return -1;
+  const unsigned *row = P.hub_bits + (long long)(lane < s ? h : 0) * P.hub_words;
+  unsigned long long R = 0;
+  for (int j = 1; j < s; j++) {
+    const int y = __shfl(x, j);
+    if (lane < j && ((row[y >> 5] >> (y & 31)) & 1u)) R |= 1ull << j;
+  }
+  unsigned long long acc = 0;
+  for (int j = 1; j < s; j++) {
+    const unsigned long long Rj = __shfl(R, j);
+    if ((R >> j) & 1ull) acc += (unsigned)__popcll(R & Rj);
+  }
+  return (long long)wave_sum(acc);
+}
+
 // Iterative DFS from stack level `floor` (its list, iterator and end already
 // set).  Stack level L iterates C(L+1) (L >= 1) or the edge pairs (L = 0);
 // taking element x at level L fixes M(L+1) = x and computes C(L+2).
@@ -1994,7 +2023,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       return;
     }
     int *out = lbase + (long long)level * P.dstride;  // list of level+1
-    if (!STATS && !P.emit_n && pl->tail_t && u == k - pl->tail_t) {
+    if (!STATS && !P.emit_n && P.tails && pl->tail_t && u == k - pl->tail_t) {
       // the tail's completions, counted from |C(u)| (timed kernel, counting)
       const unsigned long long c = (unsigned long long)extend<false, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode,
                                                                           false, balg, next);
@@ -2013,6 +2042,14 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       }
     }
     const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
+    if (!STATS && !P.emit_n && P.tails && pl->tri_tail && u == k - 3 && n <= kWave && P.hub_bits) {
+      // the path's completions are the triangles of G[C(u)] (timed kernel, counting)
+      const long long tri = n >= 3 ? triangles_small(P, out, n) : 0;
+      if (tri >= 0) {
+        cnt += (unsigned long long)tri;
+        continue;
+      }
+    }
     if (n > 0) {
       S.len[level + 1] = n;
       if (prepare_hoists<STATS, MERGE>(P, pl, S, u - 1, u - 1, u + 1, n, lbase, hbase)) {
@@ -2579,6 +2616,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->leaf_memo = 1;
   t->leaf_hash = 1;
   t->ext_hash = 1;
+  t->tails = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -2609,6 +2647,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->leaf_memo != 2) t->leaf_memo = 1;
   if (t->leaf_hash != 2 && t->leaf_hash != 3) t->leaf_hash = 1;
   if (t->ext_hash != 2) t->ext_hash = 1;
+  if (t->tails != 2) t->tails = 1;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -3038,7 +3077,41 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
   d.free_mask = 0;
   d.leaf_in = 0;
   d.tail_t = 0;
-  d.pad3_ = 0;
+  d.tri_tail = 0;
+  if (!p.labeled && p.k >= 5) {  // the triangle's first vertex a = k-3 >= 2
+    const int a = p.k - 3, b = p.k - 2, c = p.k - 1;
+    auto bnset = [&](int v) {
+      int m = 0;
+      for (int i = 0; i < p.num_bn[v]; i++) m |= 1 << p.bn[v][i];
+      return m;
+    };
+    bool ok = bnset(b) == (bnset(a) | 1 << a) && bnset(c) == (bnset(a) | 1 << a | 1 << b);
+    // less[x][y]: M(x) < M(y) in every match (transitive closure of LT / GT)
+    bool less[kMaxK][kMaxK] = {};
+    for (int v = 1; v < p.k; v++)
+      for (int j = 0; j < v; j++) {
+        if (p.cond[v][j] == TDFS_LT) less[v][j] = true;
+        if (p.cond[v][j] == TDFS_GT) less[j][v] = true;
+      }
+    for (int m = 0; m < p.k; m++)
+      for (int x = 0; x < p.k; x++)
+        for (int y = 0; y < p.k; y++)
+          if (less[x][m] && less[m][y]) less[x][y] = true;
+    for (int x = a; ok && x <= c; x++) {
+      ok = !less[x][x];  // a cycle of LT / GT admits no match
+      for (int y = x + 1; ok && y <= c; y++) ok = less[x][y] || less[y][x];  // a, b, c totally ordered
+    }
+    // every one of a, b, c stands to each earlier vertex j exactly as a's own
+    // condition on j says: the candidates of a then hold every match's
+    // M(b) and M(c) too, and every element of them meets b's and c's
+    // conditions on j
+    for (int j = 0; ok && j < a; j++) {
+      const int o = p.cond[a][j];
+      for (int v = a; ok && v <= c; v++)
+        ok = less[v][j] == (o == TDFS_LT) && less[j][v] == (o == TDFS_GT);
+    }
+    d.tri_tail = ok ? 1 : 0;
+  }
   for (int t = 2; t <= p.k - 2; t++) {  // the tail starts at u0 = k - t >= 2
     const int u0 = p.k - t;
     bool ok = true;
@@ -3061,8 +3134,10 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
         for (int a = u0; a < p.k; a++)
           for (int b = u0; b < p.k; b++)
             if (less[a][m] && less[m][b]) less[a][b] = true;
-      for (int a = u0; a < p.k && ok; a++)
+      for (int a = u0; a < p.k && ok; a++) {
+        ok = !less[a][a];  // a cycle of LT / GT admits no match
         for (int b = a + 1; b < p.k && ok; b++) ok = less[a][b] || less[b][a];
+      }
     }
     if (ok) d.tail_t = t;
   }
@@ -3292,6 +3367,7 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   P.leaf_memo = t.leaf_memo == 1 && P.leaf_batch ? 1 : 0;
   P.leaf_hash = t.leaf_hash == 1 ? 3 : (t.leaf_hash == 3 ? 1 : 0);
   P.ext_hash = t.ext_hash == 1 ? 1 : 0;
+  P.tails = t.tails == 1 ? 1 : 0;
   P.limit_ticks = t.time_limit_ms > 0 ? (long long)t.time_limit_ms * 1000 * ticks_per_us : 0;
   P.variant = wv;
   P.leaf_pair_cap = t.leaf_pair_cap;

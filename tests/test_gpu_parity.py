@@ -231,6 +231,68 @@ def test_tails_timed_kernel(dev):
     dg.free()
 
 
+def triangle_tail(plan):
+    """Unlabeled plans whose last three vertices a, b, c form a triangle over
+    one set of earlier neighbours, are totally ordered among themselves
+    (no cycle) and each stand to every earlier vertex as a's own condition
+    on it says (closure of the LT / GT conditions)."""
+    k = plan.k
+    if plan.labeled or k < 5:
+        return False
+    a, b, c = k - 3, k - 2, k - 1
+    bn = lambda v: {plan.bn[v][i] for i in range(plan.num_bn[v])}
+    if bn(b) != bn(a) | {a} or bn(c) != bn(a) | {a, b}:
+        return False
+    less = set()
+    for v in range(k):
+        for j in range(v):
+            if plan.cond[v][j] == 0:  # TDFS_LT: M(v) < M(j)
+                less.add((v, j))
+            if plan.cond[v][j] == 1:  # TDFS_GT
+                less.add((j, v))
+    for m in range(k):
+        for x in range(k):
+            for y in range(k):
+                if (x, m) in less and (m, y) in less:
+                    less.add((x, y))
+    if any((x, x) in less for x in (a, b, c)):
+        return False
+    if not all((x, y) in less or (y, x) in less for x in (a, b, c) for y in (a, b, c) if x < y):
+        return False
+    return all(((v, j) in less) == (plan.cond[a][j] == 0) and ((j, v) in less) == (plan.cond[a][j] == 1)
+               for j in range(a) for v in (a, b, c))
+
+
+def test_triangle_tail_timed_kernel(dev):
+    """Cliques K5-K7, whose last three vertices form a triangle over all the
+    earlier ones: the timed kernel counts a path's completions as the
+    triangles of its candidate set when that set has at most 64 vertices,
+    all with neighbour bitmaps; counts equal the oracle's with every vertex
+    a hub, with some (a set with a vertex without a bitmap is expanded level
+    by level), with none, with donation forced and with the 5-wave kernel."""
+    G = T.CSRGraph.chung_lu(3000, 24000, 2.1, 300, 97)
+    base = dev.tuning()
+    clique = lambda k: "".join(f"v {i} 1\n" for i in range(k)) + "".join(
+        f"e {i} {j} 1\n" for i in range(k) for j in range(i + 1, k))
+    cases = [(f"K{k}", clique(k)) for k in (5, 6, 7)]
+    assert all(triangle_tail(T.plan_pattern(t, 0)) for _, t in cases)
+    want = {name: O.count(G.rowptr, G.colidx, None, O.plan(text, 0), threads=4)["raw"] for name, text in cases}
+    dev.set_stats(False)
+    for hub in (dict(hub_min_degree=1, hub_budget_mb=4096), dict(hub_min_degree=40, hub_budget_mb=4096),
+                dict(hub_budget_mb=-1)):
+        dev.set_tuning(**hub)
+        dg = dev.upload(G)
+        for mode in (dict(), dict(timeout_us=1, donate_levels=5), dict(waves_per_simd=5), dict(tails=2)):
+            dev.set_tuning(**mode)
+            for name, text in cases:
+                say("triangle tail", name, hub, mode)
+                assert dev.count(dg, T.plan_pattern(text, 0)).raw == want[name], (name, hub, mode)
+            dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+            dev.set_tuning(**hub)
+        dg.free()
+        dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+
+
 def test_scheduling_modes_are_exact(dev):
     """Forced scheduler paths: donate at every check, a tiny task ring (full
     ring path), odd chunk sizes, reuse off; counts and algorithmic bytes are
@@ -281,6 +343,7 @@ def test_scheduling_modes_are_exact(dev):
         dict(leaf_hash=3),
         dict(leaf_hash=3, leaf_pair_cap=64, timeout_us=1),
         dict(ext_hash=2),
+        dict(tails=2),
         dict(ext_hash=2, leaf_hash=2, leaf_memo=2),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
