@@ -133,6 +133,11 @@ typedef struct tdfs_tuning {
                                 set of earlier neighbours (cliques k >= 5) as the triangles among the
                                 first one's candidates when they are at most 64 vertices with neighbour
                                 bitmaps; 2: off */
+  int32_t free_order;        /* 1 (default): the counting kernel matches a labeled pattern in an order
+                                that places a vertex as soon as its pattern neighbours are matched and no
+                                unmatched vertex shares its label, when that yields more such free levels
+                                (each enumerated for one candidate and multiplied); counts are unchanged,
+                                statistics and enumeration keep the plan's order; 2: off */
 } tdfs_tuning;
 
 /* Result of one counted pattern. */

@@ -60,6 +60,7 @@ class Tuning(C.Structure):
         ("leaf_hash", C.c_int32),
         ("ext_hash", C.c_int32),
         ("tails", C.c_int32),
+        ("free_order", C.c_int32),
     ]
 
 

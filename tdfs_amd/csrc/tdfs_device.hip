@@ -124,9 +124,13 @@ enum { kFull = 0, kHoisted = 1, kBuildH = 2, kMemo = 3 };
 // performed by the donor (statistics only).
 struct __attribute__((aligned(64))) TaskRec {
   int pid, p, a, b;
-  int fresh, pad0, pad1, pad2;
+  int fresh, pad0;
+  // the task's count stands for this many (the product of 1 + the skipped
+  // iterations of every free level it lies under)
+  unsigned long long mul;
   int M[8];
 };
+static_assert(sizeof(TaskRec) == 64, "one task per 64-byte slot");
 
 // Control words, each on its own 128-byte line.
 struct Ctrl {
@@ -254,8 +258,10 @@ struct WaveLds {
   // probe elements, their membership tests by binary search (per 64-lane
   // round and input); memoised-leaf calls
   unsigned long long ctr[8];
-  // free levels: the count when the level's first (measured) iteration began
+  // free levels: the count when the level's first (measured) iteration
+  // began, and the level's other iterations (claimed with it)
   unsigned long long mbase[8];
+  int mrest[8];
 };
 
 template <bool STATS>
@@ -1763,6 +1769,7 @@ __device__ __forceinline__ bool ring_take(const KParams &P, TaskRec &t) {
     t.a = uni(t.a);
     t.b = uni(t.b);
     t.fresh = uni(t.fresh);
+    t.mul = uni64(t.mul);
 #pragma unroll
     for (int j = 0; j < 8; j++) t.M[j] = uni(t.M[j]);
   }
@@ -1773,11 +1780,13 @@ __device__ __forceinline__ bool ring_take(const KParams &P, TaskRec &t) {
 // [floor, level], cut into about 2*idle pieces (at least split_chunk
 // iterations each, at most 256 pieces).  Ring slots are reserved first, then
 // the iterations are taken from the level (a sibling may have stolen part of
-// them meanwhile: slots left over carry empty tasks, p = 0).  Returns the
-// number of tasks pushed (0 when the ring is full).  All lanes call it.
+// them meanwhile: slots left over carry empty tasks, p = 0).  A task under
+// free levels being measured (mmask, levels below the donated one) carries
+// umul times 1 + the skipped iterations of each.  Returns the number of tasks
+// pushed (0 when the ring is full).  All lanes call it.
 template <bool STEAL>
 __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
-                      const int *L0c, int idle) {
+                      const int *L0c, int idle, int mmask, unsigned long long umul) {
   constexpr bool shared = STEAL;
   int ld = -1, rem = 0;
   for (int l = floor; l <= level; l++) {
@@ -1789,6 +1798,9 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
     }
   }
   if (ld < 0) return 0;
+  unsigned long long mul = umul;
+  for (int m = floor; m < ld; m++)
+    if ((mmask >> m) & 1) mul *= 1ull + (unsigned)uni(S.mrest[m]);
   int piece = 1, ntasks = rem;
   if (ld > 0) {
     const int want = max(1, min(256, 2 * idle));
@@ -1809,11 +1821,12 @@ __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int
   for (int j = lane_id(); j < ntasks; j += kWave) {
     TaskRec t;
     t.pid = pid;
-    t.pad0 = t.pad1 = t.pad2 = 0;
+    t.pad0 = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++) t.M[q] = 0;
     t.a = t.b = 0;
     t.fresh = 0;
+    t.mul = mul;
     if (j >= real) {
       t.p = 0;  // empty task
     } else if (ld == 0) {
@@ -1906,7 +1919,7 @@ template <bool STATS, bool STEAL, bool MERGE>
 __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int pid, int floor,
                     const int *L0r, const int *L0c, int *lbase, int *hbase, unsigned long long &cnt,
                     unsigned long long &balg, unsigned long long &next, unsigned long long &don,
-                    long long *prof, long long t_wave) {
+                    long long *prof, long long t_wave, unsigned long long umul) {
   const int k = pl->k;
   constexpr bool shared = STEAL;
   int level = floor;
@@ -1914,10 +1927,10 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   long long tprog = t0;
   const bool may_donate = P.timeout_ticks >= 0;
   // Free levels (timed kernel, counting): when vertex level+1 is free, the
-  // subtree count of the level's first iteration is measured and stands for
-  // each of its other iterations.  Nothing inside a measured subtree is
-  // donated (its count must all land here); the level's own untaken
-  // iterations may be, and are then measured by their receiver.
+  // level's other iterations are claimed with its first, whose subtree count
+  // is measured and stands for each of them.  Work donated from inside a
+  // measured subtree carries the factor (1 + skipped iterations) for its
+  // receiver to apply; the count measured here is that of the work kept.
   const int fmask = (!STATS && !STEAL && !P.emit_n) ? pl->free_mask : 0;
   int mmask = 0;  // levels whose first iteration's subtree is being measured
   // Nothing is donated, and no progress reported, before this call has
@@ -1929,9 +1942,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     if ((mmask >> level) & 1) {
       // back at a measuring level: the first iteration's subtree is complete
       const unsigned long long sub = cnt - S.mbase[level];
-      int nrest = 0;
-      (void)level_claim(S, level, kIeMax, nrest, shared);
-      cnt += sub * (unsigned long long)(unsigned)nrest;
+      cnt += sub * (unsigned long long)(unsigned)uni(S.mrest[level]);
       mmask &= ~(1 << level);
       continue;
     }
@@ -1963,8 +1974,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
         idle = uni(idle);
         if (idle > 0) {
           const long long tp = STATS ? clock64() : 0;
-          const int dl = mmask ? min(level, __builtin_ctz(mmask)) : level;
-          const int n = donate<STEAL>(P, S, pid, floor, dl, L0r, L0c, idle);
+          const int n = donate<STEAL>(P, S, pid, floor, level, L0r, L0c, idle, mmask, umul);
           if (STATS) prof[5] += clock64() - tp;
           don += (unsigned)n;
           if (n > 0) continue;  // re-read this level: its end has moved
@@ -2001,7 +2011,12 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     if (n1 == 0) continue;
     worked = true;
     if ((fmask >> (level + 1)) & 1) {  // this iteration's subtree count stands for the level's others
-      if (lane_id() == 0) S.mbase[level] = cnt;
+      int nrest = 0;
+      (void)level_claim(S, level, kIeMax, nrest, shared);
+      if (lane_id() == 0) {
+        S.mbase[level] = cnt;
+        S.mrest[level] = nrest;
+      }
       mmask |= 1 << level;
     }
     if (level == 0) {
@@ -2401,7 +2416,9 @@ __device__ __forceinline__ void match_body(const KParams &P) {
         }
       }
       if (floor >= 0)
-        dfs<STATS, STEAL, MERGE>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave);
+        dfs<STATS, STEAL, MERGE>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave,
+                                 kind == 2 ? t.mul : 1ull);
+      if (kind == 2) cnt *= t.mul;  // the task's count stands for t.mul such counts
       // a DFS cut short (time limit) leaves levels open: close them before the next unit's plan is published
       if (STEAL)
         for (int j = 0; j < kMaxK - 2; j++) level_set(S, j, 0, 0, true);
@@ -2617,6 +2634,7 @@ extern "C" void tdfs_tuning_default(tdfs_tuning *t) {
   t->leaf_hash = 1;
   t->ext_hash = 1;
   t->tails = 1;
+  t->free_order = 1;
 }
 
 static void fill_defaults(tdfs_tuning *t) {
@@ -2648,6 +2666,7 @@ static void fill_defaults(tdfs_tuning *t) {
   if (t->leaf_hash != 2 && t->leaf_hash != 3) t->leaf_hash = 1;
   if (t->ext_hash != 2) t->ext_hash = 1;
   if (t->tails != 2) t->tails = 1;
+  if (t->free_order != 2) t->free_order = 1;
 }
 
 static int launch_init(Launch &L, int device) {
@@ -3258,9 +3277,65 @@ static int graph_dstride(const tdfs_graph *g) { return std::max(64, (g->dmax + 6
 // fixed: every buffer of L is already in place (plan table of plan_cap
 // entries, ring of q_cap tasks, workspace of wbuf_ints); nothing is
 // allocated, and the grid shrinks to the workgroups whose stacks fit.
+// Free levels of a labeled plan: positions w in 2..k-3 whose vertex no later
+// vertex has as a neighbour or shares a label with.
+static int free_level_count(const tdfs_plan &p) {
+  int n = 0;
+  for (int w = 2; p.labeled && w <= p.k - 3; w++) {
+    bool f = true;
+    for (int v = w + 1; v < p.k && f; v++) f = p.adj[w][v] == 0 && p.label[v] != p.label[w];
+    n += f;
+  }
+  return n;
+}
+
+// Matching order of a labeled plan for the counting kernel.  A labeled
+// count is that of embeddings (injective only among vertices of one label),
+// which no connected order changes.  Vertices 0 and 1 (the level-0 edge) stay;
+// the rest follow the plan's order, except that a vertex whose pattern
+// neighbours are all placed and whose label no unplaced vertex carries is
+// placed as soon as that holds (while it can still sit at a position up to
+// k-3): it becomes a free level, whose subtree is enumerated once and
+// multiplied by its candidate count.  The order is taken when it has more
+// free levels and the level-0 degree filter stays exact (prune_is_sound);
+// otherwise q = p.
+static void counting_order(const tdfs_plan &p, tdfs_plan &q) {
+  q = p;
+  if (!p.labeled || p.k < 5 || !prune_is_sound(p)) return;
+  int order[kMaxK], placed = 0;
+  bool in[kMaxK] = {};
+  order[placed++] = 0;
+  order[placed++] = 1;
+  in[0] = in[1] = true;
+  while (placed < p.k) {
+    int pick = -1;
+    for (int v = 2; v < p.k && pick < 0 && placed <= p.k - 3; v++) {
+      if (in[v]) continue;
+      bool ok = true;
+      for (int u = 0; u < p.k && ok; u++) {
+        if (u == v) continue;
+        if (p.adj[v][u] && !in[u]) ok = false;  // a neighbour still to come
+        if (!in[u] && p.label[u] == p.label[v]) ok = false;
+      }
+      if (ok) pick = v;
+    }
+    for (int v = 2; v < p.k && pick < 0; v++) {
+      if (in[v]) continue;
+      for (int u = 0; u < p.k && pick < 0; u++)
+        if (in[u] && p.adj[v][u]) pick = v;
+    }
+    if (pick < 0) return;  // not connected: keep the plan
+    order[placed++] = pick;
+    in[pick] = true;
+  }
+  tdfs_plan r;
+  if (tdfs::plan_permuted(p, order, &r) != TDFS_OK) return;
+  if (prune_is_sound(r) && free_level_count(r) > free_level_count(p)) q = r;
+}
+
 static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, const tdfs_plan *plans, int nplans,
                           int64_t edge_begin, int64_t edge_end, bool stats, int blocks_cap, KParams &P,
-                          const int64_t *ranges = nullptr, bool fixed = false) {
+                          const int64_t *ranges = nullptr, bool fixed = false, bool emit = false) {
   if (!ctx || !g || !plans || nplans < 1) return fail(TDFS_E_ARG, "bad arguments");
   if (g->device != ctx->device) return fail(TDFS_E_ARG, "graph and context on different devices");
   for (int i = 0; i < nplans; i++) {
@@ -3311,7 +3386,12 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
   for (int i = 0; i < nplans; i++) all_sound = all_sound && prune_is_sound(plans[i]);
   const GraphView &V = (g->ordered && all_sound) ? g->view[1] : g->view[0];
   for (int i = 0; i < nplans; i++) {
-    to_dev_plan(plans[i], g->label_values, L.h_plans[i]);
+    // the counting kernel may take another order (the statistics follow the
+    // plan's; enumeration writes vertices in the plan's order)
+    tdfs_plan q;
+    if (!stats && !emit && t.free_order == 1) counting_order(plans[i], q);
+    else q = plans[i];
+    to_dev_plan(q, g->label_values, L.h_plans[i]);
     long long lo = edge_begin, hi = edge_end;
     if (ranges) {
       lo = std::max<int64_t>(0, ranges[2 * i]);
@@ -3932,7 +4012,7 @@ extern "C" int tdfs_enumerate(tdfs_ctx *ctx, const tdfs_graph *g, const tdfs_pla
   if (cap > (uint64_t)1 << 40) return fail(TDFS_E_ARG, "cap %llu too large", (unsigned long long)cap);
   Launch &L = ctx->main;
   KParams P;
-  int rc = launch_prepare(ctx, L, g, plan, 1, edge_begin, edge_end, false, 0, P);
+  int rc = launch_prepare(ctx, L, g, plan, 1, edge_begin, edge_end, false, 0, P, nullptr, false, true);
   if (rc) return rc;
   int *d_rows = nullptr;
   unsigned long long *d_n = nullptr;

@@ -509,6 +509,17 @@ int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colid
   out3[2] = E.capped ? 1.0 : 0.0;
   return TDFS_OK;
 }
+
+int plan_permuted(const tdfs_plan &p, const int *order, tdfs_plan *out) {
+  RawPattern rp;
+  rp.k = p.k;
+  for (int i = 0; i < p.k; i++) {
+    if (order[i] < 0 || order[i] >= p.k) return fail(TDFS_E_ARG, "bad vertex order");
+    rp.label[i] = p.label[order[i]];
+    for (int j = 0; j < p.k; j++) rp.adj[i][j] = p.adj[order[i]][order[j]];
+  }
+  return build_plan(rp, p.labeled, out);
+}
 }  // namespace tdfs
 
 using namespace tdfs;

@@ -20,6 +20,11 @@ const char *last_error();
 int estimate(int32_t n, int64_t nnz, const int64_t *rowptr, const int32_t *colidx, const int32_t *labels,
              const struct tdfs_plan *plan, int32_t nsample, uint64_t seed, double cap, double *out3,
              const int32_t *deg = nullptr, const int32_t *maxnbr = nullptr);
+
+// The plan of the same pattern with its vertices taken in the order
+// order[0..k-1] (vertex i of *out is vertex order[i] of p), built as
+// tdfs_pattern_parse builds one.
+int plan_permuted(const struct tdfs_plan &p, const int *order, struct tdfs_plan *out);
 }  // namespace tdfs
 
 #endif

@@ -175,9 +175,10 @@ def free_levels(plan):
 def test_free_levels_timed_kernel(dev):
     """Labeled patterns with free middle vertices: the timed kernel counts
     one subtree per free level and multiplies; counts equal the oracle's,
-    also with donation forced at every check (nothing inside a measured
-    subtree may be donated, and a received task is never passed on before
-    it is worked on), with one-edge chunks and with the 5-wave kernel."""
+    also with donation forced at every check (work donated from inside a
+    measured subtree carries the level's multiplier to its receiver, and a
+    received task is never passed on before it is worked on), with one-edge
+    chunks and with the 5-wave kernel."""
     G = T.CSRGraph.chung_lu(6000, 30000, 2.1, 600, 91).with_labels(4, 92)
     dg = dev.upload(G)
     base = dev.tuning()
@@ -228,6 +229,31 @@ def test_tails_timed_kernel(dev):
                 dev.set_tuning(**mode)
                 assert dev.count(dg, p).raw == want, (name, labeled, mode)
                 dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+    dg.free()
+
+
+def test_free_order_timed_kernel(dev):
+    """Labeled patterns of the 64-pattern set that the counting kernel
+    matches in another order (vertices placed as soon as their neighbours are
+    matched and no unmatched vertex shares their label, giving free levels):
+    counts equal the oracle's with the reorder on and off, with donation
+    forced and with the 5-wave kernel; the statistics kernel keeps the plan's
+    order (its algorithmic bytes equal the oracle's)."""
+    G = T.CSRGraph.chung_lu(6000, 30000, 2.1, 600, 99).with_labels(4, 100)
+    dg = dev.upload(G)
+    base = dev.tuning()
+    pats = json.loads((GOLDEN / "c4_patterns.json").read_text())["patterns"]
+    cases = [e for e in pats if e["index"] in (7, 23, 29, 43, 44, 53, 58, 63)]
+    assert len(cases) == 8 and all(e["labeled"] for e in cases)
+    for e in cases:
+        check(dev, dg, G, e["text"], 1)  # statistics and timed kernels
+        want = oracle_counts(G, device_numbering(dg, G, T.plan_pattern(e["text"], 1)), e["text"], 1)[0]["raw"]
+        dev.set_stats(False)
+        for mode in (dict(free_order=2), dict(timeout_us=1, donate_levels=5), dict(waves_per_simd=5, chunk_edges=1)):
+            say("free order", e["index"], mode)
+            dev.set_tuning(**mode)
+            assert dev.count(dg, T.plan_pattern(e["text"], 1)).raw == want, (e["index"], mode)
+            dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
     dg.free()
 
 
@@ -344,6 +370,7 @@ def test_scheduling_modes_are_exact(dev):
         dict(leaf_hash=3, leaf_pair_cap=64, timeout_us=1),
         dict(ext_hash=2),
         dict(tails=2),
+        dict(free_order=2),
         dict(ext_hash=2, leaf_hash=2, leaf_memo=2),
     ]
     for i in (1, 2, 3, 5, 7, 11, 13):
