@@ -166,6 +166,11 @@ const char *tdfs_version(void);
 int tdfs_pattern_parse(const char *text, int32_t labeled, tdfs_plan *out);
 /* Same, reading the file at path. */
 int tdfs_pattern_load(const char *path, int32_t labeled, tdfs_plan *out);
+/* The plan the counting kernel runs for this one (tdfs_tuning.free_order):
+   the same pattern in another vertex order, or a copy of it.  order (k ints,
+   may be null) receives the pattern vertex at each position.  No counterpart
+   in the reference: it exposes this library's choice of matching order. */
+int tdfs_plan_counting_order(const tdfs_plan *plan, tdfs_plan *out, int32_t *order);
 /* Matching-order construction: permutes a connected pattern so that every
  * vertex u >= 1 has a backward neighbour and vertex 2 is adjacent to vertex 1
  * (the precondition of the level-0 prune, src/gpu_match.cu:540-548).

@@ -132,6 +132,15 @@ def plan_pattern(text: str, labeled: bool) -> Plan:
     return p
 
 
+def counting_order(plan: Plan) -> tuple[Plan, list[int]]:
+    """The plan the counting kernel runs for `plan` (free_order tuning on) and
+    the pattern vertex at each of its positions."""
+    q = Plan()
+    order = np.zeros(max(plan.k, 1), dtype=np.int32)
+    check(lib().tdfs_plan_counting_order(C.byref(plan), C.byref(q), i32p(order)), "counting order")
+    return q, [int(x) for x in order[: plan.k]]
+
+
 def load_pattern(path: str, labeled: bool) -> Plan:
     p = Plan()
     check(lib().tdfs_pattern_load(str(path).encode(), int(labeled), C.byref(p)), "pattern load")

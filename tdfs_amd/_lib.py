@@ -85,6 +85,7 @@ _SIGS = {
     "tdfs_last_error": (C.c_char_p, []),
     "tdfs_version": (C.c_char_p, []),
     "tdfs_pattern_parse": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(Plan)]),
+    "tdfs_plan_counting_order": (C.c_int, [C.POINTER(Plan), C.POINTER(Plan), _I32P]),
     "tdfs_pattern_load": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(Plan)]),
     "tdfs_pattern_reorder": (C.c_int, [C.c_char_p, _I32P, C.c_char_p, C.c_size_t]),
     "tdfs_graph_read_meta": (C.c_int, [C.c_char_p, _I64P]),

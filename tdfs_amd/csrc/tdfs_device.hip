@@ -102,12 +102,14 @@ struct DevPlan {
   // N(M(b)), b in BN(k-1)) in every embedding -- j is adjacent to every such b
   int leaf_in;
   // tail: the last tail_t vertices (tail_t >= 2, 0: none) have one backward
-  // neighbour set, are pairwise non-adjacent and are either totally ordered
+  // neighbour set, are pairwise non-adjacent and either are totally ordered
   // by the symmetry conditions and otherwise bound only by injectivity
   // (unlabeled: their completions are the tail_t-subsets of C(k - tail_t))
-  // or share one label (labeled: the ordered tail_t-tuples of distinct
-  // elements of C(k - tail_t))
+  // or are labeled (their completions: for each label, the ordered tuples of
+  // distinct elements of the candidates of the label's first tail vertex, as
+  // many as the tail has vertices of that label)
   int tail_t;
+  int tail_gsz[8];  // tail vertex v first of its label in the tail: the tail's vertices of that label; else 0
   // 1: the last three vertices form a triangle over one set of earlier
   // neighbours, totally ordered and otherwise bound only by injectivity
   // (unlabeled): a path's completions are the triangles of G[C(k-3)]
@@ -814,9 +816,10 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
       return ok;
     });
   } else if (start < stop) {
-    if (leaf && nl == 1 && label_known && !P.emit_n) {
-      // Count-only leaf over one list: |list ∩ (lo,hi)| minus the excluded
-      // path vertices that lie in that window and in the list.
+    if ((leaf || (out == nullptr && mode != kBuildH && mode != kMemo)) && nl == 1 && label_known && !P.emit_n) {
+      // Count only (the leaf, a tail or a free vertex) over one list:
+      // |list ∩ (lo,hi)| minus the excluded path vertices that lie in that
+      // window and in the list.
       n = (unsigned)(stop - start);
 #pragma unroll
       for (int j = 0; j < 6; j++) {
@@ -1780,16 +1783,18 @@ __device__ __forceinline__ bool ring_take(const KParams &P, TaskRec &t) {
 // [floor, level], cut into about 2*idle pieces (at least split_chunk
 // iterations each, at most 256 pieces).  Ring slots are reserved first, then
 // the iterations are taken from the level (a sibling may have stolen part of
-// them meanwhile: slots left over carry empty tasks, p = 0).  A task under
-// free levels being measured (mmask, levels below the donated one) carries
-// umul times 1 + the skipped iterations of each.  Returns the number of tasks
-// pushed (0 when the ring is full).  All lanes call it.
+// them meanwhile: slots left over carry empty tasks, p = 0).  The stand-in
+// level of a free vertex (fmask) is never donated; a task under free levels
+// being measured (mmask, levels below the donated one) carries umul times
+// 1 + the other candidates of each.  Returns the number of tasks pushed (0
+// when the ring is full).  All lanes call it.
 template <bool STEAL>
 __device__ __forceinline__ int donate(const KParams &P, WaveLds &S, int pid, int floor, int level, const int *L0r,
-                      const int *L0c, int idle, int mmask, unsigned long long umul) {
+                      const int *L0c, int idle, int fmask, int mmask, unsigned long long umul) {
   constexpr bool shared = STEAL;
   int ld = -1, rem = 0;
   for (int l = floor; l <= level; l++) {
+    if ((fmask >> (l + 1)) & 1) continue;
     const unsigned long long w = ie_read(S, l);
     if (ie_it(w) < ie_end(w)) {
       ld = l;
@@ -1887,6 +1892,47 @@ __device__ __forceinline__ void clear_hoists(WaveLds &S) {
   for (int u = 0; u < 8; u++) S.hv[u] = 0;
 }
 
+// The completions of a path by the plan's tail, which starts at u0: the
+// product over the tail's labels (one group when unlabeled) of the ordered
+// tuples (labeled) or subsets (unlabeled) of distinct candidates of the
+// group's first vertex, as many as the group has vertices.  M(0..ufix) are
+// fixed; the vertices after ufix (a free vertex between, the tail's own)
+// stand as -1, which matches no candidate.  mode0: the extension mode of u0
+// (H(v) of a later tail vertex is this path's only when built on vertices
+// before ufix; one built on M(ufix) is a previous sibling's).  Returns false,
+// leaving f undefined, when a product would not fit 64 bits.
+template <bool MERGE>
+__device__ __forceinline__ bool tail_product(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u0,
+                                             int ufix, int mode0, const int *lbase, int *hbase,
+                                             unsigned long long &balg, unsigned long long &next,
+                                             unsigned long long &f) {
+  if (lane_id() == 0)
+    for (int j = ufix + 1; j < kMaxK; j++) S.M[j] = -1;
+  const int k = pl->k;
+  const int ri = P.reuse ? 0 : 1;
+  f = 1;
+  for (int v = u0; v < k && f; v++) {
+    const int g = pl->tail_gsz[v];
+    if (!g) continue;
+    const int mv = v == u0 ? mode0 : ((S.hv[v] && pl->hfix[ri][v] < ufix) ? kHoisted : kFull);
+    const unsigned long long c =
+        (unsigned long long)extend<false, MERGE>(P, pl, S, v, nullptr, lbase, hbase, mv, false, balg, next);
+    const unsigned long long lim =
+        g == 1 ? ~0ull : (g == 2 ? 3000000000ull : (g == 3 ? 2000000ull : (g == 4 ? 50000ull : 6000ull)));
+    if (c > lim) return false;  // c^g must fit 63 bits
+    unsigned long long x = 1;
+    for (int i = 0; i < g; i++) x *= c >= (unsigned)i ? c - i : 0ull;  // c (c-1) ... (c-g+1)
+    if (!pl->labeled) {
+      unsigned long long fac = 1;
+      for (int i = 2; i <= g; i++) fac *= (unsigned)i;
+      x /= fac;  // C(c, g)
+    }
+    if (x != 0 && f > ~0ull / x) return false;
+    f *= x;
+  }
+  return true;
+}
+
 // Triangles of G[S] for a sorted list S of at most 64 vertices that all
 // have neighbour bitmaps: lane i holds S[i] and the mask R_i of the indices
 // j > i with S[j] adjacent to it (bits of its own row: the few lines of the
@@ -1926,10 +1972,10 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   long long t0 = wall_clock64();
   long long tprog = t0;
   const bool may_donate = P.timeout_ticks >= 0;
-  // Free levels (timed kernel, counting): when vertex level+1 is free, the
-  // level's other iterations are claimed with its first, whose subtree count
-  // is measured and stands for each of them.  Work donated from inside a
-  // measured subtree carries the factor (1 + skipped iterations) for its
+  // Free levels (timed kernel, counting): a free vertex's candidates are
+  // counted, not listed; its level holds one stand-in whose subtree count is
+  // measured and stands for each candidate.  Work donated from inside a
+  // measured subtree carries the factor (the candidate count) for its
   // receiver to apply; the count measured here is that of the work kept.
   const int fmask = (!STATS && !STEAL && !P.emit_n) ? pl->free_mask : 0;
   int mmask = 0;  // levels whose first iteration's subtree is being measured
@@ -1974,7 +2020,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
         idle = uni(idle);
         if (idle > 0) {
           const long long tp = STATS ? clock64() : 0;
-          const int n = donate<STEAL>(P, S, pid, floor, level, L0r, L0c, idle, mmask, umul);
+          const int n = donate<STEAL>(P, S, pid, floor, level, L0r, L0c, idle, fmask, mmask, umul);
           if (STATS) prof[5] += clock64() - tp;
           don += (unsigned)n;
           if (n > 0) continue;  // re-read this level: its end has moved
@@ -2010,20 +2056,16 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     const int it = level_claim(S, level, 1, n1, shared);
     if (n1 == 0) continue;
     worked = true;
-    if ((fmask >> (level + 1)) & 1) {  // this iteration's subtree count stands for the level's others
-      int nrest = 0;
-      (void)level_claim(S, level, kIeMax, nrest, shared);
-      if (lane_id() == 0) {
-        S.mbase[level] = cnt;
-        S.mrest[level] = nrest;
-      }
+    const bool stand_in = (fmask >> (level + 1)) & 1;
+    if (stand_in) {  // the stand-in of a free vertex: its subtree count stands for each candidate
+      if (lane_id() == 0) S.mbase[level] = cnt;
       mmask |= 1 << level;
     }
     if (level == 0) {
       S.M[0] = L0r[it];
       S.M[1] = L0c[it];
     } else {
-      S.M[level + 1] = lbase[(long long)(level - 1) * P.dstride + it];
+      S.M[level + 1] = stand_in ? -1 : lbase[(long long)(level - 1) * P.dstride + it];
     }
     const int u = level + 2;
     const int mode = S.hv[u] ? kHoisted : kFull;
@@ -2038,23 +2080,40 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       return;
     }
     int *out = lbase + (long long)level * P.dstride;  // list of level+1
-    if (!STATS && !P.emit_n && P.tails && pl->tail_t && u == k - pl->tail_t) {
-      // the tail's completions, counted from |C(u)| (timed kernel, counting)
-      const unsigned long long c = (unsigned long long)extend<false, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode,
-                                                                          false, balg, next);
-      const int t = pl->tail_t;
-      const unsigned long long lim = t == 2 ? 3000000000ull : (t == 3 ? 2000000ull : (t == 4 ? 50000ull : 6000ull));
-      if (c <= lim) {  // n^t fits 63 bits
-        unsigned long long f = 1;
-        for (int i = 0; i < t; i++) f *= c >= (unsigned)i ? c - i : 0ull;  // n (n-1) ... (n-t+1)
-        if (!pl->labeled) {
-          unsigned long long fac = 1;
-          for (int i = 2; i <= t; i++) fac *= (unsigned)i;
-          f /= fac;  // C(n, t)
-        }
+    const bool tails = !STATS && !P.emit_n && P.tails && pl->tail_t;
+    if (tails && u == k - pl->tail_t) {
+      // the tail's completions, from the candidate counts of its first
+      // vertices of each label (timed kernel, counting)
+      unsigned long long f = 1;
+      if (tail_product<MERGE>(P, pl, S, u, u - 1, mode, lbase, hbase, balg, next, f)) {
         cnt += f;
         continue;
       }
+    }
+    if ((fmask >> u) & 1) {
+      // a free vertex: its candidates are only counted.  Right above the
+      // tail, the path's completions are that count times the tail's (taken
+      // first: it is often zero); else one stand-in (id -1: no later vertex
+      // is adjacent to it or shares its label) is enumerated, whose subtree
+      // count stands for each candidate.
+      if (tails && u + 1 == k - pl->tail_t) {
+        unsigned long long f = 1;
+        const int m1 = (S.hv[u + 1] && pl->hfix[P.reuse ? 0 : 1][u + 1] < u - 1) ? kHoisted : kFull;
+        if (tail_product<MERGE>(P, pl, S, u + 1, u - 1, m1, lbase, hbase, balg, next, f)) {
+          if (f) cnt += f * extend<false, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode, false, balg, next);
+          continue;
+        }
+      }
+      const unsigned nf = extend<false, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode, false, balg, next);
+      if (nf > 0) {
+        if (lane_id() == 0) S.mrest[level + 1] = (int)nf - 1;
+        S.len[level + 1] = 1;
+        if (prepare_hoists<STATS, MERGE>(P, pl, S, u - 1, u - 1, u + 1, (int)nf, lbase, hbase)) {
+          level_set(S, level + 1, 0, 1, shared);
+          level++;
+        }
+      }
+      continue;
     }
     const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
     if (!STATS && !P.emit_n && P.tails && pl->tri_tail && u == k - 3 && n <= kWave && P.hub_bits) {
@@ -2410,7 +2469,11 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           const int n = S.len[p - 1];
           clear_hoists(S);
           if (n > 0 && prepare_hoists<STATS, MERGE>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase)) {
-            level_set(S, p - 1, min(t.a, n), min(t.b, n), shared);
+            // a free vertex's level (only ever donated whole, with its edge)
+            // holds the one stand-in for its n candidates, as in the DFS
+            const bool stand_in = !STATS && !STEAL && !P.emit_n && ((pl->free_mask >> p) & 1);
+            if (stand_in && lane == 0) S.mrest[p - 1] = n - 1;
+            level_set(S, p - 1, stand_in ? 0 : min(t.a, n), stand_in ? 1 : min(t.b, n), shared);
             floor = p - 1;
           }
         }
@@ -3047,6 +3110,44 @@ extern "C" int tdfs_graph_estimate(const tdfs_graph *g, const tdfs_plan *plan, i
                         g->h_deg[v].data(), g->h_maxnbr[v].data());
 }
 
+// Length t >= 2 of the plan's tail (0: none): the last t vertices have one
+// backward-neighbour set and are pairwise non-adjacent, and, unlabeled, are
+// bound to earlier vertices only by injectivity and totally ordered among
+// themselves (transitive closure of LT / GT, no cycle).  The longest such
+// suffix that leaves vertices 0 and 1 out.
+static int tail_length(const tdfs_plan &p) {
+  int best = 0;
+  for (int t = 2; t <= p.k - 2; t++) {  // the tail starts at u0 = k - t >= 2
+    const int u0 = p.k - t;
+    bool ok = true;
+    for (int v = u0; v < p.k && ok; v++) {
+      ok = p.num_bn[v] == p.num_bn[u0];
+      for (int i = 0; ok && i < p.num_bn[v]; i++) ok = p.bn[v][i] == p.bn[u0][i];  // ascending lists
+      for (int w = u0; ok && w < p.k; w++) ok = w == v || p.adj[v][w] == 0;
+      for (int j = 0; ok && !p.labeled && j < u0; j++) ok = p.cond[v][j] == TDFS_NE;
+    }
+    if (ok && !p.labeled) {
+      // every pair of tail vertices ordered (transitive closure of LT / GT)
+      bool less[kMaxK][kMaxK] = {};
+      for (int v = u0; v < p.k; v++)
+        for (int j = u0; j < v; j++) {
+          if (p.cond[v][j] == TDFS_LT) less[v][j] = true;
+          if (p.cond[v][j] == TDFS_GT) less[j][v] = true;
+        }
+      for (int m = u0; m < p.k; m++)
+        for (int a = u0; a < p.k; a++)
+          for (int b = u0; b < p.k; b++)
+            if (less[a][m] && less[m][b]) less[a][b] = true;
+      for (int a = u0; a < p.k && ok; a++) {
+        ok = !less[a][a];  // a cycle of LT / GT admits no match
+        for (int b = a + 1; b < p.k && ok; b++) ok = less[a][b] || less[b][a];
+      }
+    }
+    if (ok) best = t;
+  }
+  return best;
+}
+
 // label_values: the graph's distinct labels (ascending); a labeled plan's
 // labels become their dense ids, and a label no vertex carries becomes the
 // id one past the last, which no vertex carries (the plan still runs, so its
@@ -3131,34 +3232,16 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
     }
     d.tri_tail = ok ? 1 : 0;
   }
-  for (int t = 2; t <= p.k - 2; t++) {  // the tail starts at u0 = k - t >= 2
-    const int u0 = p.k - t;
-    bool ok = true;
-    for (int v = u0; v < p.k && ok; v++) {
-      ok = p.num_bn[v] == p.num_bn[u0];
-      for (int i = 0; ok && i < p.num_bn[v]; i++) ok = p.bn[v][i] == p.bn[u0][i];  // ascending lists
-      for (int w = u0; ok && w < p.k; w++) ok = w == v || p.adj[v][w] == 0;
-      if (ok && p.labeled) ok = p.label[v] == p.label[u0];
-      for (int j = 0; ok && !p.labeled && j < u0; j++) ok = p.cond[v][j] == TDFS_NE;
+  d.tail_t = tail_length(p);
+  if (d.tail_t) {
+    const int u0 = p.k - d.tail_t;
+    for (int v = u0; v < p.k; v++) {
+      bool lead = true;
+      for (int w = u0; w < v; w++) lead = lead && (p.labeled ? p.label[w] != p.label[v] : false);
+      int g = 0;
+      for (int w = v; lead && w < p.k; w++) g += !p.labeled || p.label[w] == p.label[v];
+      d.tail_gsz[v] = g;
     }
-    if (ok && !p.labeled) {
-      // every pair of tail vertices ordered (transitive closure of LT / GT)
-      bool less[kMaxK][kMaxK] = {};
-      for (int v = u0; v < p.k; v++)
-        for (int j = u0; j < v; j++) {
-          if (p.cond[v][j] == TDFS_LT) less[v][j] = true;
-          if (p.cond[v][j] == TDFS_GT) less[j][v] = true;
-        }
-      for (int m = u0; m < p.k; m++)
-        for (int a = u0; a < p.k; a++)
-          for (int b = u0; b < p.k; b++)
-            if (less[a][m] && less[m][b]) less[a][b] = true;
-      for (int a = u0; a < p.k && ok; a++) {
-        ok = !less[a][a];  // a cycle of LT / GT admits no match
-        for (int b = a + 1; b < p.k && ok; b++) ok = less[a][b] || less[b][a];
-      }
-    }
-    if (ok) d.tail_t = t;
   }
   for (int j = 0; j + 1 < p.k; j++) {
     const int u = p.k - 1;
@@ -3277,60 +3360,109 @@ static int graph_dstride(const tdfs_graph *g) { return std::max(64, (g->dmax + 6
 // fixed: every buffer of L is already in place (plan table of plan_cap
 // entries, ring of q_cap tasks, workspace of wbuf_ints); nothing is
 // allocated, and the grid shrinks to the workgroups whose stacks fit.
-// Free levels of a labeled plan: positions w in 2..k-3 whose vertex no later
-// vertex has as a neighbour or shares a label with.
-static int free_level_count(const tdfs_plan &p) {
+// Positions 2..k-1 of a labeled plan that the counting kernel enumerates
+// one candidate at a time: all but its free levels (w <= k-3, no later
+// vertex adjacent or of the same label: one candidate's subtree stands for
+// the others) and its tail (counted from candidate counts).  *fsum: the sum
+// of the free levels' positions; *emask: bit w for each enumerated position.
+static int enumerated_levels(const tdfs_plan &p, int *fsum, int *emask) {
+  const int t = tail_length(p);
+  const int tail0 = t ? p.k - t : p.k;
   int n = 0;
-  for (int w = 2; p.labeled && w <= p.k - 3; w++) {
-    bool f = true;
+  *fsum = 0;
+  *emask = 0;
+  for (int w = 2; w < tail0; w++) {
+    bool f = w <= p.k - 3;
     for (int v = w + 1; v < p.k && f; v++) f = p.adj[w][v] == 0 && p.label[v] != p.label[w];
-    n += f;
+    if (f) {
+      *fsum += w;
+    } else {
+      n++;
+      *emask |= 1 << w;
+    }
   }
   return n;
 }
 
 // Matching order of a labeled plan for the counting kernel.  A labeled
 // count is that of embeddings (injective only among vertices of one label),
-// which no connected order changes.  Vertices 0 and 1 (the level-0 edge) stay;
-// the rest follow the plan's order, except that a vertex whose pattern
-// neighbours are all placed and whose label no unplaced vertex carries is
-// placed as soon as that holds (while it can still sit at a position up to
-// k-3): it becomes a free level, whose subtree is enumerated once and
-// multiplied by its candidate count.  The order is taken when it has more
-// free levels and the level-0 degree filter stays exact (prune_is_sound);
-// otherwise q = p.
-static void counting_order(const tdfs_plan &p, tdfs_plan &q) {
+// which no connected order changes.  Vertices 0 and 1 (the level-0 edge)
+// stay; of the connected orders of the others whose level-0 degree filter
+// stays exact (prune_is_sound) and in which every enumerated vertex keeps at
+// least the backward neighbours it has in the plan's order (no enumerated
+// candidate set grows for want of constraints), the one with the fewest
+// enumerated levels is taken when it has fewer than the plan's own -- ties go
+// to free levels at shallower positions (fewer prefixes to extend them
+// under), then to the order closest to the plan's.  Otherwise q = p.
+static void counting_order(const tdfs_plan &p, tdfs_plan &q, int *chosen = nullptr) {
   q = p;
-  if (!p.labeled || p.k < 5 || !prune_is_sound(p)) return;
-  int order[kMaxK], placed = 0;
-  bool in[kMaxK] = {};
-  order[placed++] = 0;
-  order[placed++] = 1;
-  in[0] = in[1] = true;
-  while (placed < p.k) {
-    int pick = -1;
-    for (int v = 2; v < p.k && pick < 0 && placed <= p.k - 3; v++) {
-      if (in[v]) continue;
-      bool ok = true;
-      for (int u = 0; u < p.k && ok; u++) {
-        if (u == v) continue;
-        if (p.adj[v][u] && !in[u]) ok = false;  // a neighbour still to come
-        if (!in[u] && p.label[u] == p.label[v]) ok = false;
-      }
-      if (ok) pick = v;
+  if (chosen)
+    for (int i = 0; i < p.k; i++) chosen[i] = i;
+  if (!p.labeled || p.k < 4 || !prune_is_sound(p)) return;
+  int fs0 = 0, em0 = 0;
+  const int base = enumerated_levels(p, &fs0, &em0);
+  int best = base, best_fs = 0, best_disp = 0;
+  int best_order[kMaxK] = {};
+  int rest[kMaxK];
+  for (int i = 0; i < p.k - 2; i++) rest[i] = i + 2;
+  do {
+    int order[kMaxK] = {0, 1};
+    bool conn = true;
+    for (int i = 2; i < p.k; i++) {
+      order[i] = rest[i - 2];
+      bool any = false;
+      for (int j = 0; j < i && !any; j++) any = p.adj[order[i]][order[j]] != 0;
+      conn = conn && any;
     }
-    for (int v = 2; v < p.k && pick < 0; v++) {
-      if (in[v]) continue;
-      for (int u = 0; u < p.k && pick < 0; u++)
-        if (in[u] && p.adj[v][u]) pick = v;
+    if (!conn) continue;
+    // the fields the scores read (labels, adjacency, degrees, backward
+    // neighbours); the plan itself is built for the order taken
+    tdfs_plan r;
+    std::memset(&r, 0, sizeof(r));
+    r.k = p.k;
+    r.labeled = p.labeled;
+    for (int i = 0; i < p.k; i++) {
+      r.label[i] = p.label[order[i]];
+      r.deg[i] = p.deg[order[i]];
+      for (int j = 0; j < p.k; j++) r.adj[i][j] = p.adj[order[i]][order[j]];
+      for (int j = 0; j < i; j++)
+        if (r.adj[i][j]) r.bn[i][r.num_bn[i]++] = j;
     }
-    if (pick < 0) return;  // not connected: keep the plan
-    order[placed++] = pick;
-    in[pick] = true;
-  }
+    if (!prune_is_sound(r)) continue;
+    int fs = 0, em = 0, disp = 0;
+    const int e = enumerated_levels(r, &fs, &em);
+    bool kept = true;
+    for (int i = 2; i < p.k; i++) {
+      disp += std::abs(order[i] - i);
+      if ((em >> i) & 1) kept = kept && r.num_bn[i] >= p.num_bn[order[i]];
+    }
+    if (!kept) continue;
+    if (e < best || (e == best && e < base && (fs < best_fs || (fs == best_fs && disp < best_disp)))) {
+      best = e;
+      best_fs = fs;
+      best_disp = disp;
+      std::memcpy(best_order, order, sizeof(order));
+    }
+  } while (std::next_permutation(rest, rest + p.k - 2));
   tdfs_plan r;
-  if (tdfs::plan_permuted(p, order, &r) != TDFS_OK) return;
-  if (prune_is_sound(r) && free_level_count(r) > free_level_count(p)) q = r;
+  if (best < base && tdfs::plan_permuted(p, best_order, &r) == TDFS_OK) {
+    q = r;
+    if (chosen)
+      for (int i = 0; i < p.k; i++) chosen[i] = best_order[i];
+  }
+}
+
+extern "C" int tdfs_plan_counting_order(const tdfs_plan *plan, tdfs_plan *out, int32_t *order) {
+  if (!plan || !out) return fail(TDFS_E_ARG, "null argument");
+  int rc = check_plan(*plan);
+  if (rc) return rc;
+  int chosen[kMaxK];
+  tdfs_plan q;
+  counting_order(*plan, q, chosen);
+  *out = q;
+  if (order)
+    for (int i = 0; i < plan->k; i++) order[i] = chosen[i];
+  return TDFS_OK;
 }
 
 static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, const tdfs_plan *plans, int nplans,
