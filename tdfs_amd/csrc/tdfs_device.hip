@@ -114,6 +114,14 @@ struct DevPlan {
   // neighbours, totally ordered and otherwise bound only by injectivity
   // (unlabeled): a path's completions are the triangles of G[C(k-3)]
   int tri_tail;
+  // leaf factor (counting kernel): lf_a >= 1 when this plan is a pattern's
+  // first k vertices and its dropped last vertex (kept at index k: label,
+  // backward neighbours, conditions) has candidates fixed by M(0..lf_a) up to
+  // lf_excl later vertices that always lie among them and are excluded; each
+  // iteration at level lf_a - 1 stands for that many completions of the leaf
+  // per count of the rest; -1: none
+  int lf_a;
+  int lf_excl;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -1892,6 +1900,20 @@ __device__ __forceinline__ void clear_hoists(WaveLds &S) {
   for (int u = 0; u < 8; u++) S.hv[u] = 0;
 }
 
+// Completions of the dropped leaf of a factored plan (index k of a k-vertex
+// plan) given M(0..la): its candidates, less the later vertices that always
+// lie among them and must differ from it.  Vertices after la stand as -1.
+template <bool MERGE>
+__device__ __forceinline__ long long leaf_factor_count(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S,
+                                                       int la, const int *lbase, int *hbase,
+                                                       unsigned long long &balg, unsigned long long &next) {
+  if (lane_id() == 0)
+    for (int j = la + 1; j < kMaxK; j++) S.M[j] = -1;
+  const long long c =
+      (long long)extend<false, MERGE>(P, pl, S, pl->k, nullptr, lbase, hbase, kFull, false, balg, next);
+  return c - pl->lf_excl;
+}
+
 // The completions of a path by the plan's tail, which starts at u0: the
 // product over the tail's labels (one group when unlabeled) of the ordered
 // tuples (labeled) or subsets (unlabeled) of distinct candidates of the
@@ -1978,6 +2000,7 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   // measured subtree carries the factor (the candidate count) for its
   // receiver to apply; the count measured here is that of the work kept.
   const int fmask = (!STATS && !STEAL && !P.emit_n) ? pl->free_mask : 0;
+  const int la = (!STATS && !STEAL && !P.emit_n) ? pl->lf_a : -1;
   int mmask = 0;  // levels whose first iteration's subtree is being measured
   // Nothing is donated, and no progress reported, before this call has
   // claimed work of its own: a received task could otherwise be passed on
@@ -2066,6 +2089,17 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       S.M[1] = L0c[it];
     } else {
       S.M[level + 1] = stand_in ? -1 : lbase[(long long)(level - 1) * P.dstride + it];
+    }
+    if (level + 1 == la) {
+      // the dropped leaf's completions, fixed by M(0..la): this iteration's
+      // subtree count stands for each of them
+      const long long lf = leaf_factor_count<MERGE>(P, pl, S, la, lbase, hbase, balg, next);
+      if (lf <= 0) continue;
+      if (lane_id() == 0) {
+        S.mbase[level] = cnt;
+        S.mrest[level] = (int)(lf - 1);
+      }
+      mmask |= 1 << level;
     }
     const int u = level + 2;
     const int mode = S.hv[u] ? kHoisted : kFull;
@@ -2478,10 +2512,18 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           }
         }
       }
+      unsigned long long umul = kind == 2 ? t.mul : 1ull;
+      if (floor == 1 && kind == 2 && t.fresh && !STATS && !STEAL && !P.emit_n && pl->lf_a == 1) {
+        // a donated edge (level 0, p = 2) of a factored plan: its leaf factor
+        // (the donor never claimed it; tasks from deeper levels carry theirs)
+        const long long lf = leaf_factor_count<MERGE>(P, pl, S, 1, lbase, hbase, balg, next);
+        if (lf <= 0) floor = -1;
+        else umul *= (unsigned long long)lf;
+      }
       if (floor >= 0)
         dfs<STATS, STEAL, MERGE>(P, pl, S, pid, floor, L0r, L0c, lbase, hbase, cnt, balg, next, don, prof, t_wave,
-                                 kind == 2 ? t.mul : 1ull);
-      if (kind == 2) cnt *= t.mul;  // the task's count stands for t.mul such counts
+                                 umul);
+      cnt *= umul;  // a task's count stands for umul such counts
       // a DFS cut short (time limit) leaves levels open: close them before the next unit's plan is published
       if (STEAL)
         for (int j = 0; j < kMaxK - 2; j++) level_set(S, j, 0, 0, true);
@@ -3196,6 +3238,8 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
   }
   d.free_mask = 0;
   d.leaf_in = 0;
+  d.lf_a = -1;
+  d.lf_excl = 0;
   d.tail_t = 0;
   d.tri_tail = 0;
   if (!p.labeled && p.k >= 5) {  // the triangle's first vertex a = k-3 >= 2
@@ -3465,6 +3509,66 @@ extern "C" int tdfs_plan_counting_order(const tdfs_plan *plan, tdfs_plan *out, i
   return TDFS_OK;
 }
 
+// Leaf factor of a plan for the counting kernel.  The last vertex v = k-1
+// depends on M(0..a) only, a = max(1, last backward neighbour of v) <= k-4,
+// when every later vertex j in (a, v) that v must differ from (all of them
+// unlabeled, those of v's label labeled) is adjacent to every backward
+// neighbour of v -- M(j) then always lies among v's candidates and is one of
+// them to exclude -- and, unlabeled, v has no LT / GT condition (an excluded
+// M(j) always meets v's conditions).  v's completions are then the same for
+// every completion of vertices a+1..k-2: the count of v's candidates given
+// M(0..a), less the excluded ones.  Returns a (and *excl), or -1.
+static int leaf_factor(const tdfs_plan &p, int *excl) {
+  const int v = p.k - 1;
+  if (p.k < 5 || p.num_bn[v] < 1) return -1;
+  int f = 0;
+  for (int i = 0; i < p.num_bn[v]; i++) f = std::max(f, (int)p.bn[v][i]);
+  const int a = std::max(f, 1);
+  if (a > p.k - 4) return -1;
+  for (int j = 0; !p.labeled && j < v; j++)
+    if (p.cond[v][j] != TDFS_NE) return -1;
+  int n = 0;
+  for (int j = a + 1; j < v; j++) {
+    if (p.labeled && p.label[j] != p.label[v]) continue;
+    for (int i = 0; i < p.num_bn[v]; i++)
+      if (!p.adj[j][p.bn[v][i]]) return -1;
+    n++;
+  }
+  *excl = n;
+  return a;
+}
+
+// The plan the counting kernel runs: the pattern's first k-1 vertices, with
+// the dropped leaf kept at index k-1 for the per-path factor.  Derived
+// fields are the first k-1 vertices' own, except that a free level must also
+// be free of the dropped leaf.
+static void to_dev_plan_factored(const tdfs_plan &p, int a, int excl, const std::vector<int32_t> &label_values,
+                                 DevPlan &d) {
+  DevPlan full;
+  to_dev_plan(p, label_values, full);
+  tdfs_plan r = p;
+  r.k = p.k - 1;
+  to_dev_plan(r, label_values, d);
+  const int v = p.k - 1;
+  d.label[v] = full.label[v];
+  d.deg[v] = full.deg[v];
+  d.num_bn[v] = full.num_bn[v];
+  for (int i = 0; i < 8; i++) d.bn[v][i] = full.bn[v][i];
+  d.gt_mask[v] = full.gt_mask[v];
+  d.lt_mask[v] = full.lt_mask[v];
+  d.same_label_mask[v] = full.same_label_mask[v];
+  d.dep[v] = -1;
+  d.num_bn_sh[v] = 0;
+  for (int ri = 0; ri < 2; ri++) {
+    d.hoist[ri][v] = 0;
+    d.hx_n[ri][v] = 0;
+    d.hfix[ri][v] = -1;
+  }
+  d.free_mask &= full.free_mask;
+  d.lf_a = a;
+  d.lf_excl = excl;
+}
+
 static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, const tdfs_plan *plans, int nplans,
                           int64_t edge_begin, int64_t edge_end, bool stats, int blocks_cap, KParams &P,
                           const int64_t *ranges = nullptr, bool fixed = false, bool emit = false) {
@@ -3523,7 +3627,10 @@ static int launch_prepare(const tdfs_ctx *ctx, Launch &L, const tdfs_graph *g, c
     tdfs_plan q;
     if (!stats && !emit && t.free_order == 1) counting_order(plans[i], q);
     else q = plans[i];
-    to_dev_plan(q, g->label_values, L.h_plans[i]);
+    int excl = 0;
+    const int la = (!stats && !emit && t.tails == 1 && !st) ? leaf_factor(q, &excl) : -1;
+    if (la >= 1) to_dev_plan_factored(q, la, excl, g->label_values, L.h_plans[i]);
+    else to_dev_plan(q, g->label_values, L.h_plans[i]);
     long long lo = edge_begin, hi = edge_end;
     if (ranges) {
       lo = std::max<int64_t>(0, ranges[2 * i]);

@@ -263,6 +263,51 @@ def test_free_order_timed_kernel(dev):
     dg.free()
 
 
+def leaf_factor(plan):
+    """The level after which the last vertex's completions no longer depend
+    on the path (its backward neighbours fixed, every later vertex it must
+    differ from always among its candidates, no LT / GT on it), or -1."""
+    k, v = plan.k, plan.k - 1
+    if k < 5:
+        return -1
+    bn = [plan.bn[v][i] for i in range(plan.num_bn[v])]
+    a = max(max(bn), 1)
+    if a > k - 4 or (not plan.labeled and any(plan.cond[v][j] != 2 for j in range(v))):
+        return -1
+    for j in range(a + 1, v):
+        if (not plan.labeled or plan.label[j] == plan.label[v]) and not all(plan.adj[j][b] for b in bn):
+            return -1
+    return a
+
+
+def test_leaf_factor_timed_kernel(dev):
+    """Patterns whose last vertex depends on an early prefix only (a
+    pendant, a vertex over the first edge, labeled ones over later
+    vertices): the counting kernel drops it and multiplies each prefix's
+    count by its completions; counts equal the oracle's, also with donation
+    forced at every check (donated edges and subtrees carry the factor), with
+    one-edge chunks, with the 5-wave kernel and with the factor off."""
+    G = T.CSRGraph.chung_lu(6000, 30000, 2.1, 600, 99).with_labels(4, 100)
+    dg = dev.upload(G)
+    base = dev.tuning()
+    pats = {e["index"]: e for e in json.loads((GOLDEN / "c4_patterns.json").read_text())["patterns"]}
+    cases = [(f"p{i:02d}", pats[i]["text"], pats[i]["labeled"]) for i in (2, 19, 22, 34, 39, 49, 57)]
+    cases.append(("K4 + pendant on an edge vertex",
+                  "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n"
+                  "e 0 4 1\n", False))
+    assert all(leaf_factor(T.counting_order(T.plan_pattern(t, l))[0]) >= 1 for _, t, l in cases)
+    dev.set_stats(False)
+    for name, text, labeled in cases:
+        want = O.count(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(text, labeled), threads=4)["raw"]
+        for mode in (dict(), dict(timeout_us=1, donate_levels=5), dict(chunk_edges=1, timeout_us=1),
+                     dict(waves_per_simd=5), dict(tails=2)):
+            say("leaf factor", name, mode)
+            dev.set_tuning(**mode)
+            assert dev.count(dg, T.plan_pattern(text, labeled)).raw == want, (name, mode)
+            dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
+    dg.free()
+
+
 def triangle_tail(plan):
     """Unlabeled plans whose last three vertices a, b, c form a triangle over
     one set of earlier neighbours, are totally ordered among themselves

@@ -108,6 +108,54 @@ def test_reorder_preserves_counts():
             assert got == want
 
 
+def permuted_text(plan, order):
+    k = plan.k
+    text = "".join(f"v {i} {plan.label[order[i]]}\n" for i in range(k))
+    return text + "".join(f"e {i} {j} 1\n" for i in range(k) for j in range(i + 1, k)
+                          if plan.adj[order[i]][order[j]])
+
+
+def test_counting_order_keeps_counts():
+    """The counting kernel's order for labeled plans: a permutation fixing
+    vertices 0 and 1 whose plan is the pattern's in that order; the labeled
+    count of the reordered pattern equals the plan's (oracle, and brute force
+    on a tiny graph); unlabeled plans keep their order.  The 64-pattern set's
+    labeled patterns include reordered ones."""
+    import json
+    pats = json.loads((GOLDEN / "c4_patterns.json").read_text())["patterns"]
+    rng = np.random.default_rng(31)
+    texts = [(e["text"], e["labeled"]) for e in pats]
+    for _ in range(40):
+        k = int(rng.integers(4, 8))
+        texts.append((T.gen_pattern(k, int(rng.integers(k - 1, k + 3)), 3, int(rng.integers(1 << 62))), True))
+    G = T.CSRGraph.chung_lu(400, 1600, 2.1, 60, 32).with_labels(4, 33)
+    tiny = T.CSRGraph.erdos_renyi(9, 20, 34).with_labels(3, 35)
+    moved = 0
+    for text, labeled in texts:
+        p = T.plan_pattern(text, labeled)
+        q, order = T.counting_order(p)
+        k = p.k
+        assert sorted(order) == list(range(k)) and order[:2] == [0, 1]
+        if not labeled:
+            assert order == list(range(k))
+            continue
+        for i in range(k):
+            assert q.label[i] == p.label[order[i]]
+            for j in range(k):
+                assert q.adj[i][j] == p.adj[order[i]][order[j]]
+        if order == list(range(k)):
+            continue
+        moved += 1
+        ptext = permuted_text(p, order)
+        assert T.plan_pattern(ptext, 1).adj[2][1] == q.adj[2][1]
+        want = O.count(G.rowptr, G.colidx, G.labels, O.plan(text, 1))["count"]
+        assert O.count(G.rowptr, G.colidx, G.labels, O.plan(ptext, 1))["count"] == want
+        if k <= 6:
+            assert O.bruteforce(tiny.rowptr, tiny.colidx, tiny.labels, O.plan(ptext, 1)) == \
+                O.bruteforce(tiny.rowptr, tiny.colidx, tiny.labels, O.plan(text, 1))
+    assert moved >= 11
+
+
 def test_bin_format_roundtrip(tmp_path):
     G = T.CSRGraph.chung_lu(500, 2000, 2.2, 60, 3).with_labels(4, 9)
     prefix = str(tmp_path / "g")
