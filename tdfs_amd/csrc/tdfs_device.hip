@@ -94,7 +94,7 @@ struct DevPlan {
   // involves M(k-2), so the leaf is counted for all its siblings from one
   // intersection (leaf_memo)
   int leaf_free[2];
-  // bit w (2 <= w <= k-3): vertex w is free -- no later vertex has it as a
+  // bit w (2 <= w <= k-2): vertex w is free -- no later vertex has it as a
   // backward neighbour or shares its label (labeled plans), so no later
   // candidate set depends on M(w) and every M(w) has the same subtree count
   int free_mask;
@@ -1999,7 +1999,8 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
   // measured and stands for each candidate.  Work donated from inside a
   // measured subtree carries the factor (the candidate count) for its
   // receiver to apply; the count measured here is that of the work kept.
-  const int fmask = (!STATS && !STEAL && !P.emit_n) ? pl->free_mask : 0;
+  // (a free vertex right above the leaf is only ever counted with it)
+  const int fmask = (!STATS && !STEAL && !P.emit_n) ? pl->free_mask & (P.tails ? ~0 : ~(1 << (k - 2))) : 0;
   const int la = (!STATS && !STEAL && !P.emit_n) ? pl->lf_a : -1;
   int mmask = 0;  // levels whose first iteration's subtree is being measured
   // Nothing is donated, and no progress reported, before this call has
@@ -2130,6 +2131,16 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       // first: it is often zero); else one stand-in (id -1: no later vertex
       // is adjacent to it or shares its label) is enumerated, whose subtree
       // count stands for each candidate.
+      if (!STATS && !P.emit_n && P.tails && u + 1 == k - 1) {
+        // right above the leaf: the leaf's count times this vertex's
+        if (lane_id() == 0)
+          for (int j = u; j < kMaxK; j++) S.M[j] = -1;
+        const int ml = (S.hv[k - 1] && pl->hfix[P.reuse ? 0 : 1][k - 1] < u - 1) ? kHoisted : kFull;
+        const unsigned long long f =
+            extend<false, MERGE>(P, pl, S, k - 1, nullptr, lbase, hbase, ml, false, balg, next);
+        if (f) cnt += f * extend<false, MERGE>(P, pl, S, u, nullptr, lbase, hbase, mode, false, balg, next);
+        continue;
+      }
       if (tails && u + 1 == k - pl->tail_t) {
         unsigned long long f = 1;
         const int m1 = (S.hv[u + 1] && pl->hfix[P.reuse ? 0 : 1][u + 1] < u - 1) ? kHoisted : kFull;
@@ -2505,7 +2516,7 @@ __device__ __forceinline__ void match_body(const KParams &P) {
           if (n > 0 && prepare_hoists<STATS, MERGE>(P, pl, S, 1, p - 1, p + 1, n, lbase, hbase)) {
             // a free vertex's level (only ever donated whole, with its edge)
             // holds the one stand-in for its n candidates, as in the DFS
-            const bool stand_in = !STATS && !STEAL && !P.emit_n && ((pl->free_mask >> p) & 1);
+            const bool stand_in = !STATS && !STEAL && !P.emit_n && p <= k - 3 && ((pl->free_mask >> p) & 1);
             if (stand_in && lane == 0) S.mrest[p - 1] = n - 1;
             level_set(S, p - 1, stand_in ? 0 : min(t.a, n), stand_in ? 1 : min(t.b, n), shared);
             floor = p - 1;
@@ -3293,7 +3304,7 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
     for (int i = 0; i < p.num_bn[u]; i++) in = in && p.adj[j][p.bn[u][i]] != 0;  // never for j in BN(u)
     if (in) d.leaf_in |= 1 << j;
   }
-  for (int w = 2; p.labeled && w <= p.k - 3; w++) {
+  for (int w = 2; p.labeled && w <= p.k - 2; w++) {
     bool free_ = true;
     for (int v = w + 1; v < p.k; v++) {
       if (d.label[v] == d.label[w]) free_ = false;
@@ -3405,7 +3416,7 @@ static int graph_dstride(const tdfs_graph *g) { return std::max(64, (g->dmax + 6
 // entries, ring of q_cap tasks, workspace of wbuf_ints); nothing is
 // allocated, and the grid shrinks to the workgroups whose stacks fit.
 // Positions 2..k-1 of a labeled plan that the counting kernel enumerates
-// one candidate at a time: all but its free levels (w <= k-3, no later
+// one candidate at a time: all but its free levels (w <= k-2, no later
 // vertex adjacent or of the same label: one candidate's subtree stands for
 // the others) and its tail (counted from candidate counts).  *fsum: the sum
 // of the free levels' positions; *emask: bit w for each enumerated position.
@@ -3416,7 +3427,7 @@ static int enumerated_levels(const tdfs_plan &p, int *fsum, int *emask) {
   *fsum = 0;
   *emask = 0;
   for (int w = 2; w < tail0; w++) {
-    bool f = w <= p.k - 3;
+    bool f = w <= p.k - 2;
     for (int v = w + 1; v < p.k && f; v++) f = p.adj[w][v] == 0 && p.label[v] != p.label[w];
     if (f) {
       *fsum += w;
@@ -3435,9 +3446,9 @@ static int enumerated_levels(const tdfs_plan &p, int *fsum, int *emask) {
 // stays exact (prune_is_sound) and in which every enumerated vertex keeps at
 // least the backward neighbours it has in the plan's order (no enumerated
 // candidate set grows for want of constraints), the one with the fewest
-// enumerated levels is taken when it has fewer than the plan's own -- ties go
-// to free levels at shallower positions (fewer prefixes to extend them
-// under), then to the order closest to the plan's.  Otherwise q = p.
+// enumerated levels, then with its free levels at the shallowest positions
+// (fewer prefixes to count them under), then closest to the plan's order, is
+// taken when it beats the plan's own on the first two.  Otherwise q = p.
 static void counting_order(const tdfs_plan &p, tdfs_plan &q, int *chosen = nullptr) {
   q = p;
   if (chosen)
@@ -3445,7 +3456,8 @@ static void counting_order(const tdfs_plan &p, tdfs_plan &q, int *chosen = nullp
   if (!p.labeled || p.k < 4 || !prune_is_sound(p)) return;
   int fs0 = 0, em0 = 0;
   const int base = enumerated_levels(p, &fs0, &em0);
-  int best = base, best_fs = 0, best_disp = 0;
+  int best = base, best_fs = fs0, best_disp = 0;
+  bool found = false;
   int best_order[kMaxK] = {};
   int rest[kMaxK];
   for (int i = 0; i < p.k - 2; i++) rest[i] = i + 2;
@@ -3481,7 +3493,8 @@ static void counting_order(const tdfs_plan &p, tdfs_plan &q, int *chosen = nullp
       if ((em >> i) & 1) kept = kept && r.num_bn[i] >= p.num_bn[order[i]];
     }
     if (!kept) continue;
-    if (e < best || (e == best && e < base && (fs < best_fs || (fs == best_fs && disp < best_disp)))) {
+    if (e < best || (e == best && (fs < best_fs || (fs == best_fs && found && disp < best_disp)))) {
+      found = true;
       best = e;
       best_fs = fs;
       best_disp = disp;
@@ -3489,7 +3502,7 @@ static void counting_order(const tdfs_plan &p, tdfs_plan &q, int *chosen = nullp
     }
   } while (std::next_permutation(rest, rest + p.k - 2));
   tdfs_plan r;
-  if (best < base && tdfs::plan_permuted(p, best_order, &r) == TDFS_OK) {
+  if (found && tdfs::plan_permuted(p, best_order, &r) == TDFS_OK) {
     q = r;
     if (chosen)
       for (int i = 0; i < p.k; i++) chosen[i] = best_order[i];
