@@ -112,7 +112,9 @@ struct DevPlan {
   int tail_gsz[8];  // tail vertex v first of its label in the tail: the tail's vertices of that label; else 0
   // 1: the last three vertices form a triangle over one set of earlier
   // neighbours, totally ordered and otherwise bound only by injectivity
-  // (unlabeled): a path's completions are the triangles of G[C(k-3)]
+  // (unlabeled): a path's completions are the triangles of G[C(k-3)];
+  // 2: the same, labeled, with three distinct labels: the triangles across
+  // the candidates of the three labels
   int tri_tail;
   // leaf factor (counting kernel): lf_a >= 1 when this plan is a pattern's
   // first k vertices and its dropped last vertex (kept at index k: label,
@@ -688,14 +690,17 @@ __device__ __forceinline__ unsigned merge_intersect(WaveLds &S, const int *A, in
 template <bool STATS, bool MERGE = false>
 __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u,
                            int *__restrict__ out, const int *lbase, const int *hbase, int mode, bool count_stats,
-                           unsigned long long &balg, unsigned long long &next) {
+                           unsigned long long &balg, unsigned long long &next, int lab_as = -1) {
+  // lab_as >= 0 (labeled plans, mode kFull): the candidates of label lab_as
+  // over u's inputs, distinct from the path vertices of that label
   count_op<STATS>(S, 4, 1);
   const int lane = lane_id();
   const int ri = P.reuse ? 0 : 1;
-  const int dep = P.reuse ? pl->dep[u] : -1;
+  const int dep = (P.reuse && lab_as < 0) ? pl->dep[u] : -1;
+  const int ulab = lab_as >= 0 ? lab_as : pl->label[u];
   unsigned long long bytes = 0;
   // labeled plans read only the label-l(u) segment of each adjacency list
-  const int seg = (pl->labeled && P.colidx_lab) ? pl->label[u] : -1;
+  const int seg = (pl->labeled && P.colidx_lab) ? ulab : -1;
   int minfull = INT_MAX;  // statistics: shortest full input list
   if (STATS && count_stats) {
     if (dep >= 0) {
@@ -775,7 +780,11 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
   const int gt = pl->gt_mask[u], lt = pl->lt_mask[u];
   // over label segments every candidate carries l(u): only path vertices of
   // that label can coincide with one
-  const int same = (labeled && seg >= 0) ? pl->same_label_mask[u] : 0x7f;
+  int same = (labeled && seg >= 0) ? pl->same_label_mask[u] : 0x7f;
+  if (labeled && seg >= 0 && lab_as >= 0) {
+    same = 0;
+    for (int j = 0; j < u; j++) same |= (pl->label[j] == lab_as) << j;
+  }
 #pragma unroll
   for (int j = 0; j < 6; j++) {
     nev[j] = -1;
@@ -814,7 +823,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
     if (me - ms >= P.merge_min && me - ms <= 8 * (stop - start)) mo = oi;
   }
   if (start < stop && mo >= 0) {
-    const int lab = pl->label[u];
+    const int lab = ulab;
     n = merge_intersect(S, pp + start, stop - start, S.ulp[mo] + ms, me - ms, out, [&](int x) {
       bool ok = true;
 #pragma unroll
@@ -844,7 +853,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
         }
       }
     } else {
-      const int lab = pl->label[u];
+      const int lab = ulab;
       // the longest other input without a neighbour bitmap, when it has at
       // most kHashMax elements, goes into this wave's LDS hash table (sized to
       // it): one coalesced pass over it instead of a binary search per probe
@@ -1980,6 +1989,58 @@ __device__ __forceinline__ long long triangles_small(const KParams &P, const int
   return (long long)wave_sum(acc);
 }
 
+// Labeled closing triangle a = u, b = u+1, c = u+2 of three distinct labels
+// over one set of earlier neighbours: the embeddings are the triples (x, y,
+// z) of candidates of labels l(a), l(b), l(c) over a's inputs that are
+// mutually adjacent.  The candidates of l(a) are sa[0..na); those of l(b)
+// and l(c) are listed into buf in turn; with at most 64 in all, each of them
+// with a neighbour bitmap, lane i holds one, of group g_i, and the mask R_i
+// of the others adjacent to it, and each triangle is the bit z of
+// R_x & R_y & C for x in A and y in B adjacent to x.  Returns -1 (not
+// counted) when the candidates exceed one wave or one has no bitmap.
+template <bool MERGE>
+__device__ __forceinline__ long long triangles_labeled(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S,
+                                                       int u, const int *sa, int na, int *buf, const int *lbase,
+                                                       int *hbase, unsigned long long &balg,
+                                                       unsigned long long &next) {
+  if (na == 0) return 0;
+  const int lane = lane_id();
+  int x = lane < na ? sa[lane] : 0;
+  int grp = lane < na ? 0 : 3;
+  int s = na;
+  int b0 = 0, b1 = 0;
+  for (int g = 1; g <= 2; g++) {
+    const int nb =
+        (int)extend<false, MERGE>(P, pl, S, u, buf, lbase, hbase, kFull, false, balg, next, pl->label[u + g]);
+    if (nb == 0) return 0;
+    if (s + nb > kWave) return -1;
+    if (lane >= s && lane < s + nb) {
+      x = buf[lane - s];
+      grp = g;
+    }
+    if (g == 1) {
+      b0 = s;
+      b1 = s + nb;
+    }
+    s += nb;
+  }
+  const int h = lane < s ? P.hub_idx[x] : 0;
+  if (__ballot(lane < s && h < 0)) return -1;
+  const unsigned *row = P.hub_bits + (long long)(lane < s ? h : 0) * P.hub_words;
+  unsigned long long R = 0;
+  for (int j = 0; j < s; j++) {
+    const int y = __shfl(x, j);
+    if (lane < s && j != lane && ((row[y >> 5] >> (y & 31)) & 1u)) R |= 1ull << j;
+  }
+  const unsigned long long Cm = __ballot(grp == 2);
+  unsigned long long acc = 0;
+  for (int j = b0; j < b1; j++) {
+    const unsigned long long Rj = __shfl(R, j);
+    if (grp == 0 && ((R >> j) & 1ull)) acc += (unsigned)__popcll(R & Rj & Cm);
+  }
+  return (long long)wave_sum(acc);
+}
+
 // Iterative DFS from stack level `floor` (its list, iterator and end already
 // set).  Stack level L iterates C(L+1) (L >= 1) or the edge pairs (L = 0);
 // taking element x at level L fixes M(L+1) = x and computes C(L+2).
@@ -2162,8 +2223,13 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
     }
     const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
     if (!STATS && !P.emit_n && P.tails && pl->tri_tail && u == k - 3 && n <= kWave && P.hub_bits) {
-      // the path's completions are the triangles of G[C(u)] (timed kernel, counting)
-      const long long tri = n >= 3 ? triangles_small(P, out, n) : 0;
+      // the path's completions are the triangles of G[C(u)] (unlabeled) or
+      // of the candidates of the three labels (labeled; listed into the next
+      // level's free list) (timed kernel, counting)
+      const long long tri = pl->tri_tail == 2
+                                ? triangles_labeled<MERGE>(P, pl, S, u, out, n, lbase + (long long)(level + 1) * P.dstride,
+                                                           lbase, hbase, balg, next)
+                                : (n >= 3 ? triangles_small(P, out, n) : 0);
       if (tri >= 0) {
         cnt += (unsigned long long)tri;
         continue;
@@ -3286,6 +3352,18 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
         ok = less[v][j] == (o == TDFS_LT) && less[j][v] == (o == TDFS_GT);
     }
     d.tri_tail = ok ? 1 : 0;
+  }
+  if (p.labeled && p.k >= 5) {
+    // labeled: a triangle over one set of earlier neighbours, three labels
+    const int a = p.k - 3, b = p.k - 2, c = p.k - 1;
+    auto bnset = [&](int v) {
+      int m = 0;
+      for (int i = 0; i < p.num_bn[v]; i++) m |= 1 << p.bn[v][i];
+      return m;
+    };
+    const bool ok = bnset(b) == (bnset(a) | 1 << a) && bnset(c) == (bnset(a) | 1 << a | 1 << b) &&
+                    p.label[a] != p.label[b] && p.label[a] != p.label[c] && p.label[b] != p.label[c];
+    d.tri_tail = ok ? 2 : 0;
   }
   d.tail_t = tail_length(p);
   if (d.tail_t) {

@@ -343,17 +343,24 @@ def triangle_tail(plan):
 def test_triangle_tail_timed_kernel(dev):
     """Cliques K5-K7, whose last three vertices form a triangle over all the
     earlier ones: the timed kernel counts a path's completions as the
-    triangles of its candidate set when that set has at most 64 vertices,
-    all with neighbour bitmaps; counts equal the oracle's with every vertex
-    a hub, with some (a set with a vertex without a bitmap is expanded level
-    by level), with none, with donation forced and with the 5-wave kernel."""
-    G = T.CSRGraph.chung_lu(3000, 24000, 2.1, 300, 97)
+    triangles of its candidate set (unlabeled) or across the candidates of
+    the last three labels (labeled, three distinct labels; some shared with
+    earlier vertices) when those are at most 64 vertices, all with neighbour
+    bitmaps; counts equal the oracle's with every vertex a hub, with some (a
+    set with a vertex without a bitmap is expanded level by level), with
+    none, with donation forced and with the 5-wave kernel."""
+    G = T.CSRGraph.chung_lu(3000, 24000, 2.1, 300, 97).with_labels(4, 98)
     base = dev.tuning()
-    clique = lambda k: "".join(f"v {i} 1\n" for i in range(k)) + "".join(
-        f"e {i} {j} 1\n" for i in range(k) for j in range(i + 1, k))
-    cases = [(f"K{k}", clique(k)) for k in (5, 6, 7)]
-    assert all(triangle_tail(T.plan_pattern(t, 0)) for _, t in cases)
-    want = {name: O.count(G.rowptr, G.colidx, None, O.plan(text, 0), threads=4)["raw"] for name, text in cases}
+    clique = lambda labs: "".join(f"v {i} {l}\n" for i, l in enumerate(labs)) + "".join(
+        f"e {i} {j} 1\n" for i in range(len(labs)) for j in range(i + 1, len(labs)))
+    cases = [(f"K{k}", clique([1] * k), 0) for k in (5, 6, 7)]
+    assert all(triangle_tail(T.plan_pattern(t, 0)) for _, t, _l in cases)
+    # labeled: last three of distinct labels (repeating earlier ones), and a
+    # control whose last two share a label (expanded level by level)
+    cases += [("K5 labeled", clique([0, 1, 2, 3, 0]), 1), ("K6 labeled", clique([1, 1, 2, 3, 0, 2]), 1),
+              ("K5 labeled, last two alike", clique([0, 1, 2, 3, 3]), 1)]
+    want = {name: O.count(G.rowptr, G.colidx, G.labels if lab else None, O.plan(text, lab), threads=4)["raw"]
+            for name, text, lab in cases}
     dev.set_stats(False)
     for hub in (dict(hub_min_degree=1, hub_budget_mb=4096), dict(hub_min_degree=40, hub_budget_mb=4096),
                 dict(hub_budget_mb=-1)):
@@ -361,9 +368,9 @@ def test_triangle_tail_timed_kernel(dev):
         dg = dev.upload(G)
         for mode in (dict(), dict(timeout_us=1, donate_levels=5), dict(waves_per_simd=5), dict(tails=2)):
             dev.set_tuning(**mode)
-            for name, text in cases:
+            for name, text, lab in cases:
                 say("triangle tail", name, hub, mode)
-                assert dev.count(dg, T.plan_pattern(text, 0)).raw == want[name], (name, hub, mode)
+                assert dev.count(dg, T.plan_pattern(text, lab)).raw == want[name], (name, hub, mode)
             dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
             dev.set_tuning(**hub)
         dg.free()
