@@ -3656,6 +3656,10 @@ static void to_dev_plan_factored(const tdfs_plan &p, int a, int excl, const std:
     d.hfix[ri][v] = -1;
   }
   d.free_mask &= full.free_mask;
+  // the factor is taken when M(a) is fixed: a shortcut that counts vertex a
+  // itself (a tail or closing triangle starting at or before a) would skip it
+  if (d.tail_t && p.k - 1 - d.tail_t <= a) d.tail_t = 0;
+  if (d.tri_tail && p.k - 1 - 3 <= a) d.tri_tail = 0;
   d.lf_a = a;
   d.lf_excl = excl;
 }

@@ -286,12 +286,19 @@ def test_leaf_factor_timed_kernel(dev):
     vertices): the counting kernel drops it and multiplies each prefix's
     count by its completions; counts equal the oracle's, also with donation
     forced at every check (donated edges and subtrees carry the factor), with
-    one-edge chunks, with the 5-wave kernel and with the factor off."""
+    one-edge chunks, with the 5-wave kernel and with the factor off.  Every
+    vertex has a neighbour bitmap, so closing-triangle shortcuts are taken
+    wherever they apply (never across the vertex the factor is taken at)."""
     G = T.CSRGraph.chung_lu(6000, 30000, 2.1, 600, 99).with_labels(4, 100)
-    dg = dev.upload(G)
     base = dev.tuning()
+    dev.set_tuning(hub_min_degree=1, hub_budget_mb=4096)
+    dg = dev.upload(G)
+    dev.set_tuning(**{f: getattr(base, f) for f, _ in base._fields_})
     pats = {e["index"]: e for e in json.loads((GOLDEN / "c4_patterns.json").read_text())["patterns"]}
     cases = [(f"p{i:02d}", pats[i]["text"], pats[i]["labeled"]) for i in (2, 19, 22, 34, 39, 49, 57)]
+    k5_apex = "".join(f"v {i} {l}\n" for i, l in enumerate([3, 0, 2, 1, 0, 1])) + "".join(
+        f"e {a} {b} 1\n" for a, b in [(i, j) for i in range(5) for j in range(i + 1, 5)] + [(0, 5), (1, 5), (2, 5)])
+    cases.append(("K5 + apex over its first triangle (labeled)", k5_apex, True))
     cases.append(("K4 + pendant on an edge vertex",
                   "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n"
                   "e 0 4 1\n", False))
