@@ -1964,22 +1964,32 @@ __device__ __forceinline__ bool tail_product(const KParams &P, const DevPlan *__
   return true;
 }
 
-// Triangles of G[S] for a sorted list S of at most 64 vertices that all
-// have neighbour bitmaps: lane i holds S[i] and the mask R_i of the indices
-// j > i with S[j] adjacent to it (bits of its own row: the few lines of the
-// row that S's ids fall in), and each triangle i < j < z is counted once as a
-// bit of R_i & R_j, j in R_i.  Returns -1 (not counted) when some element has
-// no bitmap.
+// Whether candidate x (this lane's: its neighbour-bitmap row, or its
+// adjacency list) and y (hub index hy, or -1) are adjacent: a bit of x's row,
+// else a bit of y's row, else a binary search of x's list.
+__device__ __forceinline__ bool adjacent_pair(const KParams &P, const unsigned *row, const int *lst, int len, int x,
+                                              int y, int hy) {
+  if (row) return (row[y >> 5] >> (y & 31)) & 1u;
+  if (hy >= 0) return (P.hub_bits[(long long)hy * P.hub_words + (x >> 5)] >> (x & 31)) & 1u;
+  return contains(lst, len, y);
+}
+
+// Triangles of G[S] for a list S of at most 64 vertices: lane i holds S[i]
+// and the mask R_i of the indices j > i with S[j] adjacent to it (from the
+// bitmap row of either one, else a search of S[i]'s list), and each triangle
+// i < j < z is counted once as a bit of R_i & R_j, j in R_i.
 __device__ __forceinline__ long long triangles_small(const KParams &P, const int *Sl, int s) {
   const int lane = lane_id();
   const int x = lane < s ? Sl[lane] : 0;
-  const int h = lane < s ? P.hub_idx[x] : 0;
-  if (__ballot(lane < s && h < 0)) return -1;
-  const unsigned *row = P.hub_bits + (long long)(lane < s ? h : 0) * P.hub_words;
+  const int h = (lane < s && P.hub_idx) ? P.hub_idx[x] : -1;
+  const unsigned *row = h >= 0 ? P.hub_bits + (long long)h * P.hub_words : nullptr;
+  int len = 0;
+  const int *lst = (lane < s && h < 0) ? adj_list(P, x, -1, len) : nullptr;
   unsigned long long R = 0;
   for (int j = 1; j < s; j++) {
     const int y = __shfl(x, j);
-    if (lane < j && ((row[y >> 5] >> (y & 31)) & 1u)) R |= 1ull << j;
+    const int hy = __shfl(h, j);
+    if (lane < j && adjacent_pair(P, row, lst, len, x, y, hy)) R |= 1ull << j;
   }
   unsigned long long acc = 0;
   for (int j = 1; j < s; j++) {
@@ -1993,11 +2003,11 @@ __device__ __forceinline__ long long triangles_small(const KParams &P, const int
 // over one set of earlier neighbours: the embeddings are the triples (x, y,
 // z) of candidates of labels l(a), l(b), l(c) over a's inputs that are
 // mutually adjacent.  The candidates of l(a) are sa[0..na); those of l(b)
-// and l(c) are listed into buf in turn; with at most 64 in all, each of them
-// with a neighbour bitmap, lane i holds one, of group g_i, and the mask R_i
-// of the others adjacent to it, and each triangle is the bit z of
-// R_x & R_y & C for x in A and y in B adjacent to x.  Returns -1 (not
-// counted) when the candidates exceed one wave or one has no bitmap.
+// and l(c) are listed into buf in turn; with at most 64 in all, lane i holds
+// one, of group g_i, and the mask R_i of the others adjacent to it (as in
+// triangles_small), and each triangle is the bit z of R_x & R_y & C for x in
+// A and y in B adjacent to x.  Returns -1 (not counted) when the candidates
+// exceed one wave.
 template <bool MERGE>
 __device__ __forceinline__ long long triangles_labeled(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S,
                                                        int u, const int *sa, int na, int *buf, const int *lbase,
@@ -2024,13 +2034,15 @@ __device__ __forceinline__ long long triangles_labeled(const KParams &P, const D
     }
     s += nb;
   }
-  const int h = lane < s ? P.hub_idx[x] : 0;
-  if (__ballot(lane < s && h < 0)) return -1;
-  const unsigned *row = P.hub_bits + (long long)(lane < s ? h : 0) * P.hub_words;
+  const int h = (lane < s && P.hub_idx) ? P.hub_idx[x] : -1;
+  const unsigned *row = h >= 0 ? P.hub_bits + (long long)h * P.hub_words : nullptr;
+  int len = 0;
+  const int *lst = (lane < s && h < 0) ? adj_list(P, x, -1, len) : nullptr;
   unsigned long long R = 0;
   for (int j = 0; j < s; j++) {
     const int y = __shfl(x, j);
-    if (lane < s && j != lane && ((row[y >> 5] >> (y & 31)) & 1u)) R |= 1ull << j;
+    const int hy = __shfl(h, j);
+    if (lane < s && j != lane && adjacent_pair(P, row, lst, len, x, y, hy)) R |= 1ull << j;
   }
   const unsigned long long Cm = __ballot(grp == 2);
   unsigned long long acc = 0;
@@ -2222,14 +2234,14 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       continue;
     }
     const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
-    if (!STATS && !P.emit_n && P.tails && pl->tri_tail && u == k - 3 && n <= kWave && P.hub_bits) {
+    if (!STATS && !P.emit_n && P.tails && pl->tri_tail && u == k - 3 && n <= kWave) {
       // the path's completions are the triangles of G[C(u)] (unlabeled) or
       // of the candidates of the three labels (labeled; listed into the next
       // level's free list) (timed kernel, counting)
       const long long tri = pl->tri_tail == 2
                                 ? triangles_labeled<MERGE>(P, pl, S, u, out, n, lbase + (long long)(level + 1) * P.dstride,
                                                            lbase, hbase, balg, next)
-                                : (n >= 3 ? triangles_small(P, out, n) : 0);
+                                : (n >= 3 ? triangles_small(P, out, n) : 0);  // -1: not counted here
       if (tri >= 0) {
         cnt += (unsigned long long)tri;
         continue;

@@ -352,10 +352,10 @@ def test_triangle_tail_timed_kernel(dev):
     earlier ones: the timed kernel counts a path's completions as the
     triangles of its candidate set (unlabeled) or across the candidates of
     the last three labels (labeled, three distinct labels; some shared with
-    earlier vertices) when those are at most 64 vertices, all with neighbour
-    bitmaps; counts equal the oracle's with every vertex a hub, with some (a
-    set with a vertex without a bitmap is expanded level by level), with
-    none, with donation forced and with the 5-wave kernel."""
+    earlier vertices) when those are at most 64 vertices (adjacency from
+    either vertex's bitmap row, else a search of a list); counts equal the
+    oracle's with every vertex a hub, with some, with none, with donation
+    forced and with the 5-wave kernel."""
     G = T.CSRGraph.chung_lu(3000, 24000, 2.1, 300, 97).with_labels(4, 98)
     base = dev.tuning()
     clique = lambda labs: "".join(f"v {i} {l}\n" for i, l in enumerate(labs)) + "".join(
