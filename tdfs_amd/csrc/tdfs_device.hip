@@ -124,6 +124,11 @@ struct DevPlan {
   // per count of the rest; -1: none
   int lf_a;
   int lf_excl;
+  // tx_w >= 2: the vertex right above the tail, which no tail vertex has as a
+  // backward neighbour and which the tail's group led by tx_lead must differ
+  // from (its label; unlabeled: the whole tail, NE only); -1: none
+  int tx_w;
+  int tx_lead;
 };
 
 // Modes of extend(): C(u) from the plan's inputs (kFull) or from H(u)
@@ -690,9 +695,11 @@ __device__ __forceinline__ unsigned merge_intersect(WaveLds &S, const int *A, in
 template <bool STATS, bool MERGE = false>
 __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u,
                            int *__restrict__ out, const int *lbase, const int *hbase, int mode, bool count_stats,
-                           unsigned long long &balg, unsigned long long &next, int lab_as = -1) {
+                           unsigned long long &balg, unsigned long long &next, int lab_as = -1,
+                           const int *xl = nullptr, int xn = 0) {
   // lab_as >= 0 (labeled plans, mode kFull): the candidates of label lab_as
-  // over u's inputs, distinct from the path vertices of that label
+  // over u's inputs, distinct from the path vertices of that label; xl (a
+  // sorted list of xn vertices of l(u)): one more input, intersected too
   count_op<STATS>(S, 4, 1);
   const int lane = lane_id();
   const int ri = P.reuse ? 0 : 1;
@@ -753,6 +760,7 @@ __device__ __forceinline__ unsigned extend(const KParams &P, const DevPlan *__re
     const int nb = pl->num_bn[u];
     for (int i = 0; i < nb; i++) put_adj(pl->bn[u][i]);
   }
+  if (xl) put_list(xl, xn);
   // probe = shortest list
   int pidx = 0;
   int pn = S.uln[0];
@@ -1932,17 +1940,21 @@ __device__ __forceinline__ long long leaf_factor_count(const KParams &P, const D
 // (H(v) of a later tail vertex is this path's only when built on vertices
 // before ufix; one built on M(ufix) is a previous sibling's).  Returns false,
 // leaving f undefined, when a product would not fit 64 bits.
+// With gx >= 0 (the first tail vertex of one label), f1 receives the same
+// product with one candidate of gx's group taken away.
 template <bool MERGE>
 __device__ __forceinline__ bool tail_product(const KParams &P, const DevPlan *__restrict__ pl, WaveLds &S, int u0,
                                              int ufix, int mode0, const int *lbase, int *hbase,
                                              unsigned long long &balg, unsigned long long &next,
-                                             unsigned long long &f) {
+                                             unsigned long long &f, int gx = -1,
+                                             unsigned long long *f1 = nullptr) {
   if (lane_id() == 0)
     for (int j = ufix + 1; j < kMaxK; j++) S.M[j] = -1;
   const int k = pl->k;
   const int ri = P.reuse ? 0 : 1;
   f = 1;
-  for (int v = u0; v < k && f; v++) {
+  unsigned long long h = 1;
+  for (int v = u0; v < k && (f || h); v++) {
     const int g = pl->tail_gsz[v];
     if (!g) continue;
     const int mv = v == u0 ? mode0 : ((S.hv[v] && pl->hfix[ri][v] < ufix) ? kHoisted : kFull);
@@ -1951,16 +1963,21 @@ __device__ __forceinline__ bool tail_product(const KParams &P, const DevPlan *__
     const unsigned long long lim =
         g == 1 ? ~0ull : (g == 2 ? 3000000000ull : (g == 3 ? 2000000ull : (g == 4 ? 50000ull : 6000ull)));
     if (c > lim) return false;  // c^g must fit 63 bits
-    unsigned long long x = 1;
-    for (int i = 0; i < g; i++) x *= c >= (unsigned)i ? c - i : 0ull;  // c (c-1) ... (c-g+1)
-    if (!pl->labeled) {
-      unsigned long long fac = 1;
+    unsigned long long fac = 1;
+    if (!pl->labeled)
       for (int i = 2; i <= g; i++) fac *= (unsigned)i;
-      x /= fac;  // C(c, g)
-    }
-    if (x != 0 && f > ~0ull / x) return false;
+    auto ff = [&](unsigned long long n) {  // n (n-1) ... (n-g+1), or C(n, g) unlabeled
+      unsigned long long x = 1;
+      for (int i = 0; i < g; i++) x *= n >= (unsigned)i ? n - i : 0ull;
+      return x / fac;
+    };
+    const unsigned long long x = ff(c);
+    const unsigned long long x1 = v == gx ? (c ? ff(c - 1) : 0ull) : x;
+    if ((x != 0 && f > ~0ull / x) || (x1 != 0 && h > ~0ull / x1)) return false;
     f *= x;
+    h *= x1;
   }
+  if (f1) *f1 = gx >= 0 ? h : f;
   return true;
 }
 
@@ -2234,6 +2251,22 @@ __device__ __forceinline__ void dfs(const KParams &P, const DevPlan *__restrict_
       continue;
     }
     const int n = (int)extend<STATS, MERGE>(P, pl, S, u, out, lbase, hbase, mode, true, balg, next);
+    if (tails && u == pl->tx_w && n > 0) {
+      // the tail right below u depends on M(u) only through one group's
+      // injectivity: over the n candidates x of u, the completions are
+      // P0 for x outside that group's candidates and P1 (one fewer) for
+      // the m inside (timed kernel, counting)
+      unsigned long long p0 = 0, p1 = 0;
+      S.len[level + 1] = n;  // C(u), for a tail vertex that starts from it (result reuse)
+      if (tail_product<MERGE>(P, pl, S, u + 1, u - 1, kFull, lbase, hbase, balg, next, p0, pl->tx_lead, &p1)) {
+        if (p0 | p1) {
+          const unsigned long long m = extend<false, MERGE>(P, pl, S, pl->tx_lead, nullptr, lbase, hbase, kFull, false,
+                                                            balg, next, -1, out, n);
+          cnt += ((unsigned long long)n - m) * p0 + m * p1;
+        }
+        continue;
+      }
+    }
     if (!STATS && !P.emit_n && P.tails && pl->tri_tail && u == k - 3 && n <= kWave) {
       // the path's completions are the triangles of G[C(u)] (unlabeled) or
       // of the candidates of the three labels (labeled; listed into the next
@@ -3329,6 +3362,8 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
   d.leaf_in = 0;
   d.lf_a = -1;
   d.lf_excl = 0;
+  d.tx_w = -1;
+  d.tx_lead = -1;
   d.tail_t = 0;
   d.tri_tail = 0;
   if (!p.labeled && p.k >= 5) {  // the triangle's first vertex a = k-3 >= 2
@@ -3386,6 +3421,19 @@ static void to_dev_plan(const tdfs_plan &p, const std::vector<int32_t> &label_va
       int g = 0;
       for (int w = v; lead && w < p.k; w++) g += !p.labeled || p.label[w] == p.label[v];
       d.tail_gsz[v] = g;
+    }
+    const int w = u0 - 1;
+    int lead = -1;
+    bool ok = w >= 2;
+    for (int v = u0; ok && v < p.k; v++) {
+      ok = p.adj[v][w] == 0;
+      if (!p.labeled) ok = ok && p.cond[v][w] == TDFS_NE;
+      else if (lead < 0 && p.label[v] == p.label[w]) lead = v;
+    }
+    if (!p.labeled) lead = u0;
+    if (ok && lead >= 0) {
+      d.tx_w = w;
+      d.tx_lead = lead;
     }
   }
   for (int j = 0; j + 1 < p.k; j++) {
@@ -3671,6 +3719,7 @@ static void to_dev_plan_factored(const tdfs_plan &p, int a, int excl, const std:
   // the factor is taken when M(a) is fixed: a shortcut that counts vertex a
   // itself (a tail or closing triangle starting at or before a) would skip it
   if (d.tail_t && p.k - 1 - d.tail_t <= a) d.tail_t = 0;
+  if (!d.tail_t || d.tx_w <= a) d.tx_w = -1;
   if (d.tri_tail && p.k - 1 - 3 <= a) d.tri_tail = 0;
   d.lf_a = a;
   d.lf_excl = excl;

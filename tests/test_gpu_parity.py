@@ -224,9 +224,15 @@ def test_tails_timed_kernel(dev):
     # centre's label)
     star4_mixed = "v 0 1\nv 1 0\nv 2 1\nv 3 0\ne 0 1 1\ne 0 2 1\ne 0 3 1\n"
     star5_mixed = "v 0 0\nv 1 1\nv 2 0\nv 3 1\nv 4 0\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 0 4 1\n"
+    # a tail (4, 5 over 0, 1) below a vertex (3, over 0 and 2) it must differ
+    # from: counted over 3's candidates with one exclusion (labeled: 3 shares
+    # the tail's label; unlabeled: NE only)
+    above = "e 0 1 1\ne 0 2 1\ne 1 2 1\ne 0 3 1\ne 2 3 1\ne 0 4 1\ne 1 4 1\ne 0 5 1\ne 1 5 1\n"
+    tail_excl = "v 0 1\nv 1 1\nv 2 0\nv 3 1\nv 4 1\nv 5 1\n" + above
     dev.set_stats(False)
     for name, text in (("star4", star4), ("book3", book3), ("k23", k23), ("triangle+2 pendants", tri_tail),
-                       ("star4 mixed labels", star4_mixed), ("star5 mixed labels", star5_mixed)):
+                       ("star4 mixed labels", star4_mixed), ("star5 mixed labels", star5_mixed),
+                       ("tail below a vertex it must differ from", tail_excl)):
         for labeled in (1,) if "mixed" in name else (0, 1):
             p = T.plan_pattern(text, labeled)
             want = O.count(G.rowptr, G.colidx, G.labels if labeled else None, O.plan(text, labeled), threads=4)["raw"]
