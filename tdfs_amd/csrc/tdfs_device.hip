@@ -3556,7 +3556,8 @@ static int graph_dstride(const tdfs_graph *g) { return std::max(64, (g->dmax + 6
 // Positions 2..k-1 of a labeled plan that the counting kernel enumerates
 // one candidate at a time: all but its free levels (w <= k-2, no later
 // vertex adjacent or of the same label: one candidate's subtree stands for
-// the others) and its tail (counted from candidate counts).  *fsum: the sum
+// the others), its tail (counted from candidate counts) and the vertex right
+// above the tail when no tail vertex is adjacent to it (summed over).  *fsum: the sum
 // of the free levels' positions; *emask: bit w for each enumerated position.
 static int enumerated_levels(const tdfs_plan &p, int *fsum, int *emask) {
   const int t = tail_length(p);
@@ -3567,7 +3568,10 @@ static int enumerated_levels(const tdfs_plan &p, int *fsum, int *emask) {
   for (int w = 2; w < tail0; w++) {
     bool f = w <= p.k - 2;
     for (int v = w + 1; v < p.k && f; v++) f = p.adj[w][v] == 0 && p.label[v] != p.label[w];
-    if (f) {
+    // right above the tail and adjacent to none of it: summed over, not enumerated
+    bool x = t && w == tail0 - 1;
+    for (int v = tail0; v < p.k && x; v++) x = p.adj[w][v] == 0;
+    if (f || x) {
       *fsum += w;
     } else {
       n++;
